@@ -1,0 +1,161 @@
+/*
+ * pygcmc_hip.h -- C ABI of the MI355X GCMC hot path (libpygcmc_hip.so).
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success or a
+ * negative status; gcmc_last_error() returns the message of the last failure
+ * on the calling thread.
+ */
+#ifndef PYGCMC_HIP_H_
+#define PYGCMC_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Struct ABI of pyGCMC, byte for byte (gcmc/cpp/gcmc.h:47-100):
+ * Atom 20 B, AtomArray 436 B, InfoStruct 76 B, residue 20 B. */
+typedef struct Atom {
+    float position[3];
+    float charge;
+    int type;
+} Atom;
+
+typedef struct AtomArray {
+    char name[4];
+    int startRes;
+    float muex;
+    float conc;
+    int confBias;
+    float mcTime;
+    int totalNum;
+    int maxNum;
+    int num_atoms;
+    Atom atoms[20];
+} AtomArray;
+
+typedef struct InfoStruct {
+    int mcsteps;
+    float cutoff;
+    float grid_dx;
+    float startxyz[3];
+    float cryst[3];
+    int showInfo;
+    float cavityFactor;
+    int fragTypeNum;
+    int totalGridNum;
+    int totalResNum;
+    int totalAtomNum;
+    int ffXNum;
+    int ffYNum;
+    int PME;
+    unsigned int seed;
+} InfoStruct;
+
+typedef struct residue {
+    float position[3];
+    int atomNum;
+    int atomStart;
+} residue;
+
+#define GCMC_OK 0
+#define GCMC_E_ARG (-1)
+#define GCMC_E_HIP (-2)
+#define GCMC_E_STATE (-3)
+#define GCMC_E_DEVICE (-4)
+
+/* ---- drop-in entry --------------------------------------------------------
+ * Replaces runGCMC_cuda (gcmc/cpp/gcmc.cu:21-118, declared at
+ * gcmc/cpp/gcmc.cpp:159).  Same seven arguments and the same in-place
+ * contract: runs info->mcsteps moves of moveArray and writes back
+ * fragmentInfo[].totalNum, residueInfo[] and atomInfo[].  The reference
+ * returns void; this returns a status instead of crashing. */
+int runGCMC_hip(const InfoStruct *info, AtomArray *fragmentInfo, residue *residueInfo,
+                Atom *atomInfo, const float *grid, const float *ff, const int *moveArray);
+
+const char *gcmc_last_error(void);
+
+/* ---- engine: many independent systems ("replicas") on one GPU ------------
+ * The reference runs one system per call (gcmc/cpp/gcmc.cu:21); these entry
+ * points split that call into create / add / run / download so that many
+ * systems of uneven size share one persistent kernel. */
+typedef struct gcmc_engine gcmc_engine;
+
+typedef struct gcmc_tuning {
+    float cell_size;     /* target edge (A) of the group-centre cell list; 0 = 5 A */
+    int persistent_wgs;  /* workgroups of the batched kernel; 0 = 4 per CU */
+    int bucket_cap;      /* mobile groups per cell before overflow; 0 = auto */
+    int pair_lds;        /* 1 = stage the LJ pair table in LDS when it fits */
+} gcmc_tuning;
+
+#define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */
+#define GCMC_MODE_BATCHED 1 /* all systems in one persistent kernel */
+#define GCMC_MODE_GRAPH 2   /* BATCHED, launched from a captured hipGraph */
+
+int gcmc_engine_create(gcmc_engine **out, const gcmc_tuning *tuning);
+int gcmc_engine_destroy(gcmc_engine *e);
+/* Adds one system (arrays as for runGCMC_hip; nmoves entries of moveArray).
+ * Returns the replica index (>= 0) or a negative status. */
+int gcmc_engine_add_system(gcmc_engine *e, const InfoStruct *info, const AtomArray *frag,
+                           const residue *res, const Atom *atoms, const float *grid,
+                           const float *ff, const int *moves, int nmoves);
+/* Copies all systems to the GPU (call once after the last add). */
+int gcmc_engine_upload(gcmc_engine *e);
+/* Advances every replica by nsteps moves (fewer at the end of its schedule). */
+int gcmc_engine_run(gcmc_engine *e, int nsteps, int mode);
+int gcmc_engine_synchronize(gcmc_engine *e);
+/* Current step index of a replica. */
+int gcmc_engine_step(gcmc_engine *e, int replica);
+/* Writes the replica's state back into caller arrays laid out as at add time. */
+int gcmc_engine_download(gcmc_engine *e, int replica, AtomArray *frag, residue *res,
+                         Atom *atoms);
+/* Running FP64 sum of accepted energy changes and per-species move counters
+ * (attempts/accepts: fragTypeNum*4 entries, index species*4 + kind). */
+int gcmc_engine_stats(gcmc_engine *e, int replica, double *energy, long long *attempts,
+                      long long *accepts);
+/* Milliseconds of the last gcmc_engine_run measured with HIP events. */
+int gcmc_engine_last_run_ms(gcmc_engine *e, double *ms);
+
+/* ---- step trace (parity tests) --------------------------------------------
+ * Per step s (relative to the step when tracing was enabled) and trial t:
+ * centre, energies (insert/delete: e[0]=E; translate/rotate: E_old, E_new),
+ * included-group count and set hash per evaluation, pose hash, slot, and per
+ * selection group g the picked trial and accept flag. */
+typedef struct gcmc_trace {
+    int max_steps, max_k;
+    int *k, *ngroups, *naccept, *n_after;
+    double *etot;
+    float *centre;      /* [S][K][3] */
+    double *e;          /* [S][K][2] */
+    int *ninc;          /* [S][K][2] */
+    uint32_t *inchash;  /* [S][K][2] */
+    uint32_t *posehash; /* [S][K]    */
+    int *idx;           /* [S][K]    */
+    int *gpick;         /* [S][K]    */
+    int *gacc;          /* [S][K]    */
+} gcmc_trace;
+
+int gcmc_engine_set_trace(gcmc_engine *e, int replica, int max_steps, int max_k);
+int gcmc_engine_get_trace(gcmc_engine *e, int replica, gcmc_trace *out);
+
+/* ---- single evaluations ---------------------------------------------------
+ * Energies (kJ/mol, FP32 pair terms, compensated sums, FP64 result) of npose
+ * poses of one species against the replica's current state, each tested
+ * around centres[i] and excluding residue exclude[i] (-1: none).  The same
+ * kernel path as the MC step (calcEnergy, gcmc/cpp/gcmc_energy.h:142-204). */
+int gcmc_engine_eval_poses(gcmc_engine *e, int replica, int species, int npose,
+                           const float *centres, const float *poses, const int *exclude,
+                           double *energy, int *ninc, uint32_t *inchash);
+
+/* Host execution of the engine's FP32 group-inclusion test -- the code the
+ * GPU runs, compiled for the CPU -- for n pairs of points: out[i] = 1 when
+ * distanceP(a_i, b_i, cryst) <= cutoff (gcmc/cpp/gcmc_energy.h:115-122). */
+int gcmc_host_within(int n, const float *a, const float *b, const float *cryst, float cutoff,
+                     unsigned char *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYGCMC_HIP_H_ */

@@ -1,0 +1,869 @@
+// engine.hip -- host side of the MI355X GCMC engine and its C ABI
+// (include/pygcmc_hip.h).
+//
+// Replaces runGCMC_cuda (gcmc/cpp/gcmc.cu:21-118): instead of cudaMalloc +
+// copies + a host loop with synchronous copies around every launch, systems
+// are uploaded once into a structure-of-arrays layout with a cell list over
+// group centres, and moves run with device-side decisions, either one move per
+// launch pair (plain) or every system in one persistent kernel (batched).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pygcmc_hip.h"
+#include "engine_kernels.h"
+
+static_assert(sizeof(Atom) == 20, "Atom ABI");
+static_assert(sizeof(AtomArray) == 436, "AtomArray ABI");
+static_assert(sizeof(InfoStruct) == 76, "InfoStruct ABI");
+static_assert(sizeof(residue) == 20, "residue ABI");
+
+using namespace gcmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPX(expr)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(GCMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+uint32_t fbits(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b;
+}
+float bfloat(uint32_t b)
+{
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// smallest non-negative float d with fast_round(d / L) >= m
+float shift_threshold(float L, int m)
+{
+    uint32_t lo = 0, hi = fbits(8.0f * L);
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        float d = bfloat(mid);
+        if (fast_round_ref(d / L) >= (float)m) hi = mid;
+        else lo = mid + 1;
+    }
+    return bfloat(lo);
+}
+
+// largest float x with sqrtf(x) <= rc
+float r2_threshold(float rc)
+{
+    float x = rc * rc;
+    while (x > 0.0f && sqrtf(x) > rc) x = std::nextafter(x, 0.0f);
+    while (true) {
+        float nx = std::nextafter(x, INFINITY);
+        if (sqrtf(nx) <= rc) x = nx;
+        else break;
+    }
+    return x;
+}
+
+struct DevBuf {
+    std::vector<void *> ptrs;
+    template <class T>
+    hipError_t alloc(T **p, size_t n)
+    {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) {
+            ptrs.push_back(q);
+            *p = (T *)q;
+        }
+        return e;
+    }
+    void release()
+    {
+        for (void *p : ptrs) hipFree(p);
+        ptrs.clear();
+    }
+};
+
+struct HostRep {
+    InfoStruct info;
+    std::vector<AtomArray> frag;
+    std::vector<residue> res;
+    std::vector<Atom> atoms;
+    std::vector<float> grid, ff;
+    std::vector<int> moves;
+    Rep rep{};
+    DevBuf buf;
+    DevBuf trbuf;
+    int maxk = 1;
+    int host_step = 0;
+};
+
+}  // namespace
+
+struct gcmc_engine {
+    gcmc_tuning tune{};
+    std::vector<HostRep *> reps;
+    bool uploaded = false;
+    Rep *d_rep = nullptr;
+    State *d_state = nullptr;
+    Sched *d_sched = nullptr;
+    LdsLayout lay{};
+    bool staged = false;
+    int nwg = 0;
+    int maxk = 1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_nsteps = -1;
+    int spin_limit = 1 << 22;  // polls without progress anywhere (~seconds)
+};
+
+namespace {
+
+int check_system(const InfoStruct *info, const AtomArray *frag, int nmoves)
+{
+    if (!info || !frag) return fail(GCMC_E_ARG, "null system pointer");
+    if (info->fragTypeNum < 1 || info->fragTypeNum > MAXT)
+        return fail(GCMC_E_ARG, "fragTypeNum must be in [1, 32]");
+    if (info->cryst[0] <= 0 || info->cryst[1] <= 0 || info->cryst[2] <= 0)
+        return fail(GCMC_E_ARG, "box edges must be positive");
+    if (info->totalGridNum < 1) return fail(GCMC_E_ARG, "empty cavity grid");
+    if (nmoves < 0) return fail(GCMC_E_ARG, "negative move count");
+    for (int s = 0; s < info->fragTypeNum; ++s) {
+        const AtomArray &F = frag[s];
+        if (F.num_atoms < 1 || F.num_atoms > MAXF)
+            return fail(GCMC_E_ARG, "fragment atom count must be in [1, 20]");
+        if (F.confBias < 1 || F.confBias > MAXK)
+            return fail(GCMC_E_ARG, "confBias must be in [1, 2048]");
+        if (F.totalNum < 0 || F.totalNum > F.maxNum)
+            return fail(GCMC_E_ARG, "totalNum outside [0, maxNum]");
+        if (F.startRes < 0 || F.startRes + F.maxNum > info->totalResNum)
+            return fail(GCMC_E_ARG, "species slots outside residueInfo");
+    }
+    return 0;
+}
+
+// cell stencil: offsets whose cell box lies within rc + eps of the centre cell
+std::vector<int> make_stencil(const float L[3], const int nc[3], float rc, float eps)
+{
+    int lo[3], hi[3];
+    bool full[3];
+    double h[3];
+    for (int i = 0; i < 3; ++i) {
+        h[i] = (double)L[i] / nc[i];
+        int r = (int)std::floor((rc + eps) / h[i]) + 1;
+        full[i] = (2 * r + 1 >= nc[i]);
+        if (full[i]) {
+            lo[i] = -((nc[i] - 1) / 2);
+            hi[i] = lo[i] + nc[i] - 1;
+        } else {
+            lo[i] = -r;
+            hi[i] = r;
+        }
+    }
+    const double lim = (double)(rc + eps) * (rc + eps);
+    std::vector<int> out;
+    for (int a = lo[0]; a <= hi[0]; ++a)
+        for (int b = lo[1]; b <= hi[1]; ++b)
+            for (int c = lo[2]; c <= hi[2]; ++c) {
+                int d[3] = {a, b, c};
+                double s = 0;
+                for (int i = 0; i < 3; ++i) {
+                    int n = std::abs(d[i]);
+                    if (full[i]) n = std::min(n, nc[i] - n);
+                    double g = std::max(0, n - 1) * h[i];
+                    s += g * g;
+                }
+                if (s <= lim) out.push_back((a + 128) | ((b + 128) << 8) | ((c + 128) << 16));
+            }
+    return out;
+}
+
+template <class T>
+hipError_t put(T **dst, DevBuf &b, const T *src, size_t n)
+{
+    hipError_t e = b.alloc(dst, n);
+    if (e != hipSuccess) return e;
+    if (n) e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+int upload_rep(gcmc_engine *E, HostRep &H)
+{
+    const InfoStruct &I = H.info;
+    Rep &R = H.rep;
+    R = Rep{};
+    const int T = I.fragTypeNum;
+    for (int i = 0; i < 3; ++i) {
+        R.L[i] = I.cryst[i];
+        R.start[i] = I.startxyz[i];
+        R.invL[i] = 1.0f / I.cryst[i];
+        R.ax[i].L = I.cryst[i];
+        R.ax[i].t[0] = shift_threshold(I.cryst[i], 1);
+        R.ax[i].t[1] = shift_threshold(I.cryst[i], 2);
+        R.ax[i].t[2] = shift_threshold(I.cryst[i], 3);
+    }
+    R.rc = I.cutoff;
+    R.r2max = r2_threshold(I.cutoff);
+    R.dx = I.grid_dx;
+    R.cav = I.cavityFactor;
+    R.G = I.totalGridNum;
+    R.T = T;
+    R.X = I.ffXNum;
+    R.Y = I.ffYNum;
+    R.nfix = H.frag[0].startRes;
+    R.nres = I.totalResNum;
+    R.natom = I.totalAtomNum;
+    R.nmoves = (int)H.moves.size();
+    R.seed = I.seed;
+    R.maxk = H.maxk;
+
+    // cell list geometry
+    const float hcell = E->tune.cell_size > 0 ? E->tune.cell_size : 5.0f;
+    for (int i = 0; i < 3; ++i) {
+        R.nc[i] = std::max(1, (int)std::floor(I.cryst[i] / hcell));
+        R.nc[i] = std::min(R.nc[i], 255);
+        R.inv_h[i] = (float)R.nc[i] / I.cryst[i];
+    }
+    R.ncell = R.nc[0] * R.nc[1] * R.nc[2];
+    const float h3 = (I.cryst[0] / R.nc[0]) * (I.cryst[1] / R.nc[1]) * (I.cryst[2] / R.nc[2]);
+    int cap = E->tune.bucket_cap;
+    if (cap <= 0) {
+        int want = (int)std::ceil(0.05 * h3 * 1.5);
+        cap = 8;
+        while (cap < want) cap <<= 1;
+    }
+    R.cap = cap;
+    int nmob = 0;
+    for (int s = 0; s < T; ++s) nmob += H.frag[s].maxNum;
+    R.ovf_base = R.ncell * cap;
+
+    // structure-of-arrays copies
+    std::vector<float4> res_pos(I.totalResNum);
+    std::vector<int2> res_rng(I.totalResNum);
+    for (int g = 0; g < I.totalResNum; ++g) {
+        const residue &r = H.res[g];
+        res_pos[g] = make_float4(r.position[0], r.position[1], r.position[2], 0.f);
+        res_rng[g] = make_int2(r.atomStart, r.atomNum);
+    }
+    std::vector<float4> atom(I.totalAtomNum);
+    std::vector<int> atype(I.totalAtomNum);
+    // every atom the energy kernel can read must name a column of the pair table
+    for (int g = 0; g < I.totalResNum; ++g) {
+        bool active = g < R.nfix;
+        for (int s = 0; s < T && !active; ++s)
+            active = g >= H.frag[s].startRes && g < H.frag[s].startRes + H.frag[s].totalNum;
+        if (!active) continue;
+        const residue &r = H.res[g];
+        if (r.atomStart < 0 || r.atomNum < 0 || r.atomStart + r.atomNum > I.totalAtomNum)
+            return fail(GCMC_E_ARG, "residue atom range outside atomInfo");
+        for (int a = r.atomStart; a < r.atomStart + r.atomNum; ++a)
+            if (H.atoms[a].type < 0 || H.atoms[a].type >= R.Y)
+                return fail(GCMC_E_ARG, "atom type outside the pair table columns");
+    }
+    for (int s = 0; s < T; ++s)
+        for (int j = 0; j < H.frag[s].maxNum; ++j) {
+            const residue &r = H.res[H.frag[s].startRes + j];
+            if (r.atomNum != H.frag[s].num_atoms || r.atomStart < 0 ||
+                r.atomStart + r.atomNum > I.totalAtomNum)
+                return fail(GCMC_E_ARG, "fragment slot with a bad atom range");
+        }
+    for (int a = 0; a < I.totalAtomNum; ++a) {
+        const Atom &A = H.atoms[a];
+        atom[a] = make_float4(A.position[0], A.position[1], A.position[2], A.charge);
+        atype[a] = A.type;
+    }
+    std::vector<float2> pair((size_t)R.X * R.Y);
+    for (size_t i = 0; i < pair.size(); ++i) {
+        double sg = H.ff[2 * i], ep = H.ff[2 * i + 1];
+        pair[i] = make_float2((float)(sg * sg * 100.0), (float)(4.0 * ep));
+    }
+    std::vector<Species> sp(T);
+    for (int s = 0; s < T; ++s) {
+        const AtomArray &F = H.frag[s];
+        Species &S = sp[s];
+        std::memset(&S, 0, sizeof(S));
+        S.startRes = F.startRes;
+        S.maxNum = F.maxNum;
+        S.nf = F.num_atoms;
+        S.conf = F.confBias;
+        double nbar = (double)I.cryst[0] * (double)I.cryst[1] * (double)I.cryst[2] *
+                      (double)F.conc * 0.0006023;
+        S.B = BETA * (double)F.muex + std::log(nbar);
+        for (int a = 0; a < F.num_atoms; ++a) {
+            const Atom &A = F.atoms[a];
+            S.tmpl[a] = make_float4(A.position[0], A.position[1], A.position[2], A.charge);
+            S.kq[a] = (float)(KCOUL * (double)A.charge);
+            S.trow[a] = A.type * R.Y;
+            S.ttype[a] = A.type;
+            if (A.type < 0 || A.type >= R.X)
+                return fail(GCMC_E_ARG, "fragment atom type outside the pair table rows");
+        }
+    }
+    // fixed groups: CSR by cell
+    std::vector<int> fcell(R.nfix);
+    std::vector<int> fstart(R.ncell + 1, 0);
+    for (int g = 0; g < R.nfix; ++g) {
+        const residue &r = H.res[g];
+        fcell[g] = cell_index(R.L, R.start, R.inv_h, R.nc, r.position[0], r.position[1], r.position[2]);
+        fstart[fcell[g] + 1] += 1;
+    }
+    for (int c = 0; c < R.ncell; ++c) fstart[c + 1] += fstart[c];
+    std::vector<float4> frec(std::max(R.nfix, 1));
+    {
+        std::vector<int> fill(fstart.begin(), fstart.end() - 1);
+        for (int g = 0; g < R.nfix; ++g) {
+            const residue &r = H.res[g];
+            int bits = g;
+            float w;
+            std::memcpy(&w, &bits, 4);
+            frec[fill[fcell[g]]++] = make_float4(r.position[0], r.position[1], r.position[2], w);
+        }
+    }
+    // mobile groups: buckets + overflow
+    std::vector<int> mcnt(R.ncell + 1, 0);
+    std::vector<int> mslot((size_t)R.ovf_base + std::max(nmob, 1), -1);
+    std::vector<int> mloc(I.totalResNum, -1);
+    for (int s = 0; s < T; ++s) {
+        const AtomArray &F = H.frag[s];
+        for (int j = 0; j < F.totalNum; ++j) {
+            int g = F.startRes + j;
+            const residue &r = H.res[g];
+            int c = cell_index(R.L, R.start, R.inv_h, R.nc, r.position[0], r.position[1], r.position[2]);
+            int pos;
+            if (mcnt[c] < cap) pos = c * cap + mcnt[c]++;
+            else pos = R.ovf_base + mcnt[R.ncell]++;
+            mslot[pos] = g;
+            mloc[g] = pos;
+        }
+    }
+    std::vector<int> st = make_stencil(R.L, R.nc, R.rc, 0.05f);
+    R.nst = (int)st.size();
+
+    DevBuf &B = H.buf;
+    HIPX(put((float **)&R.grid, B, H.grid.data(), H.grid.size()));
+    HIPX(put((int **)&R.moves, B, H.moves.data(), H.moves.size()));
+    HIPX(put(&R.res_pos, B, res_pos.data(), res_pos.size()));
+    HIPX(put(&R.res_rng, B, res_rng.data(), res_rng.size()));
+    HIPX(put(&R.atom, B, atom.data(), atom.size()));
+    HIPX(put(&R.atype, B, atype.data(), atype.size()));
+    HIPX(put((float2 **)&R.pair, B, pair.data(), pair.size()));
+    HIPX(put((Species **)&R.sp, B, sp.data(), sp.size()));
+    HIPX(put((int **)&R.fix_start, B, fstart.data(), fstart.size()));
+    HIPX(put((float4 **)&R.fix_rec, B, frec.data(), frec.size()));
+    HIPX(put(&R.mob_cnt, B, mcnt.data(), mcnt.size()));
+    HIPX(put(&R.mob_slot, B, mslot.data(), mslot.size()));
+    HIPX(put(&R.mob_loc, B, mloc.data(), mloc.size()));
+    HIPX(put((int **)&R.stencil, B, st.data(), st.size()));
+    HIPX(B.alloc(&R.tc, H.maxk));
+    HIPX(B.alloc(&R.tslot, H.maxk));
+    HIPX(B.alloc(&R.E, 2 * (size_t)H.maxk));
+    HIPX(B.alloc(&R.pose, (size_t)H.maxk * MAXF));
+    HIPX(B.alloc(&R.adj, (size_t)H.maxk * ADJW));
+    return 0;
+}
+
+int compute_layout(gcmc_engine *E)
+{
+    int nst = 1, xy = 1;
+    for (HostRep *H : E->reps) {
+        nst = std::max(nst, H->rep.nst);
+        xy = std::max(xy, H->rep.X * H->rep.Y);
+    }
+    const int pair_bytes = 8 * xy;
+    E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
+    auto up = [](int x) { return (x + 15) & ~15; };
+    LdsLayout &l = E->lay;
+    int o = 0;
+    l.scell = o; o = up(o + 4 * (nst + 2));
+    l.sfix = o; o = up(o + 4 * (nst + 2));
+    l.soff = o; o = up(o + 4 * (nst + 2));
+    l.incg = o; o = up(o + 4 * (INC_CAP + 1));
+    l.incs = o; o = up(o + 4 * (INC_CAP + 1));
+    l.incn = o; o = up(o + 4 * (INC_CAP + 1));
+    l.frag = o; o = up(o + 16 * MAXF);
+    l.frow = o; o = up(o + 4 * MAXF);
+    if (E->staged) {
+        l.pair = o;
+        o = up(o + pair_bytes);
+    } else {
+        l.pair = -1;
+    }
+    const int eval_end = o;
+    int a = 0;
+    l.members = a; a = up(a + 4 * E->maxk);
+    l.w = a; a = up(a + 8 * E->maxk);
+    l.accl = a; a = up(a + 8 * E->maxk);
+    l.J = 0;  // prep runs after accept in the same workgroup
+    o = std::max(eval_end, a);
+    l.misc = o; o = up(o + 4 * 32);
+    l.dred = o; o = up(o + 8 * 2 * NWAVE);
+    l.total = o;
+    if (l.total > 65536)
+        return fail(GCMC_E_ARG, "LDS layout exceeds 64 KiB (stencil or confBias too large; raise cell_size)");
+    return 0;
+}
+
+template <bool S>
+void launch_plain_step(gcmc_engine *E, int r)
+{
+    const size_t lds = E->lay.total;
+    hipLaunchKernelGGL(k_eval_plain<S>, dim3(2 * E->reps[r]->maxk), dim3(WG), lds, E->stream,
+                       E->d_rep, E->d_state, r, E->lay);
+    hipLaunchKernelGGL(k_accept_plain<S>, dim3(1), dim3(WG), lds, E->stream, E->d_rep, E->d_state, r,
+                       E->lay);
+}
+
+template <bool S>
+void launch_batched(gcmc_engine *E, int nsteps)
+{
+    const int R = (int)E->reps.size();
+    const size_t lds = E->lay.total;
+    hipLaunchKernelGGL(k_begin, dim3((R + 255) / 256), dim3(256), 0, E->stream, E->d_state, E->d_rep, 0, R,
+                       nsteps, E->d_sched);
+    hipLaunchKernelGGL(k_start<S>, dim3(R), dim3(WG), lds, E->stream, E->d_rep, E->d_state, E->d_sched,
+                       E->lay, 0, 1);
+    hipLaunchKernelGGL(k_batched<S>, dim3(E->nwg), dim3(WG), lds, E->stream, E->d_rep, E->d_state, R,
+                       E->d_sched, E->lay, E->spin_limit);
+}
+
+template <bool S>
+void launch_plain(gcmc_engine *E, int nsteps)
+{
+    const size_t lds = E->lay.total;
+    const int R = (int)E->reps.size();
+    for (int r = 0; r < R; ++r) {
+        hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, E->stream, E->d_state, E->d_rep, r, 1, nsteps,
+                           E->d_sched);
+        hipLaunchKernelGGL(k_start<S>, dim3(1), dim3(WG), lds, E->stream, E->d_rep, E->d_state, E->d_sched,
+                           E->lay, r, 0);
+        const int rem = E->reps[r]->rep.nmoves - E->reps[r]->host_step;
+        const int n = std::min(nsteps, std::max(rem, 0));
+        for (int s = 0; s < n; ++s) launch_plain_step<S>(E, r);
+    }
+}
+
+int read_states(gcmc_engine *E, std::vector<State> &st)
+{
+    st.resize(E->reps.size());
+    HIPX(hipMemcpy(st.data(), E->d_state, st.size() * sizeof(State), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sync_rep_struct(gcmc_engine *E, int r)
+{
+    if (!E->uploaded) return 0;
+    HIPX(hipMemcpy(E->d_rep + r, &E->reps[r]->rep, sizeof(Rep), hipMemcpyHostToDevice));
+    if (E->gexec) {
+        hipGraphExecDestroy(E->gexec);
+        hipGraphDestroy(E->graph);
+        E->gexec = nullptr;
+        E->graph = nullptr;
+        E->graph_nsteps = -1;
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gcmc_last_error(void) { return g_err.c_str(); }
+
+int gcmc_engine_create(gcmc_engine **out, const gcmc_tuning *tuning)
+{
+    if (!out) return fail(GCMC_E_ARG, "null engine pointer");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(GCMC_E_DEVICE, "no HIP device available");
+    gcmc_engine *E = new gcmc_engine();
+    if (tuning) E->tune = *tuning;
+    else E->tune.pair_lds = 1;
+    hipError_t e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&E->ev1);
+    if (e != hipSuccess) {
+        delete E;
+        return fail(GCMC_E_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
+    }
+    *out = E;
+    return 0;
+}
+
+int gcmc_engine_destroy(gcmc_engine *E)
+{
+    if (!E) return 0;
+    if (E->stream) hipStreamSynchronize(E->stream);
+    if (E->gexec) hipGraphExecDestroy(E->gexec);
+    if (E->graph) hipGraphDestroy(E->graph);
+    for (HostRep *H : E->reps) {
+        H->buf.release();
+        H->trbuf.release();
+        delete H;
+    }
+    if (E->d_rep) hipFree(E->d_rep);
+    if (E->d_state) hipFree(E->d_state);
+    if (E->d_sched) hipFree(E->d_sched);
+    if (E->ev0) hipEventDestroy(E->ev0);
+    if (E->ev1) hipEventDestroy(E->ev1);
+    if (E->stream) hipStreamDestroy(E->stream);
+    delete E;
+    return 0;
+}
+
+int gcmc_engine_add_system(gcmc_engine *E, const InfoStruct *info, const AtomArray *frag,
+                           const residue *res, const Atom *atoms, const float *grid, const float *ff,
+                           const int *moves, int nmoves)
+{
+    if (!E) return fail(GCMC_E_ARG, "null engine");
+    if (E->uploaded) return fail(GCMC_E_STATE, "systems must be added before upload");
+    int rc = check_system(info, frag, nmoves);
+    if (rc) return rc;
+    if (!res || !atoms || !grid || !ff || (nmoves > 0 && !moves))
+        return fail(GCMC_E_ARG, "null array pointer");
+    HostRep *H = new HostRep();
+    H->info = *info;
+    H->frag.assign(frag, frag + info->fragTypeNum);
+    H->res.assign(res, res + info->totalResNum);
+    H->atoms.assign(atoms, atoms + info->totalAtomNum);
+    H->grid.assign(grid, grid + 3 * (size_t)info->totalGridNum);
+    H->ff.assign(ff, ff + 2 * (size_t)info->ffXNum * info->ffYNum);
+    H->moves.assign(moves, moves + nmoves);
+    for (int m : H->moves) {
+        if (m < 0 || m / 4 >= info->fragTypeNum) {
+            delete H;
+            return fail(GCMC_E_ARG, "moveArray entry names a species outside fragTypeNum");
+        }
+    }
+    for (const AtomArray &F : H->frag) H->maxk = std::max(H->maxk, F.confBias);
+    E->maxk = std::max(E->maxk, H->maxk);
+    E->reps.push_back(H);
+    return (int)E->reps.size() - 1;
+}
+
+int gcmc_engine_upload(gcmc_engine *E)
+{
+    if (!E) return fail(GCMC_E_ARG, "null engine");
+    if (E->uploaded) return fail(GCMC_E_STATE, "already uploaded");
+    if (E->reps.empty()) return fail(GCMC_E_STATE, "no systems added");
+    for (HostRep *H : E->reps) {
+        int rc = upload_rep(E, *H);
+        if (rc) return rc;
+    }
+    int rc = compute_layout(E);
+    if (rc) return rc;
+    const int R = (int)E->reps.size();
+    HIPX(hipMalloc(&E->d_rep, R * sizeof(Rep)));
+    HIPX(hipMalloc(&E->d_state, R * sizeof(State)));
+    HIPX(hipMalloc(&E->d_sched, sizeof(Sched)));
+    std::vector<Rep> reps(R);
+    std::vector<State> st(R);
+    for (int r = 0; r < R; ++r) {
+        reps[r] = E->reps[r]->rep;
+        State &S = st[r];
+        std::memset(&S, 0, sizeof(S));
+        for (int s = 0; s < E->reps[r]->info.fragTypeNum; ++s) S.totalNum[s] = E->reps[r]->frag[s].totalNum;
+    }
+    HIPX(hipMemcpy(E->d_rep, reps.data(), R * sizeof(Rep), hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(E->d_state, st.data(), R * sizeof(State), hipMemcpyHostToDevice));
+    Sched z{};
+    HIPX(hipMemcpy(E->d_sched, &z, sizeof(z), hipMemcpyHostToDevice));
+    int dev = 0, ncu = 0;
+    HIPX(hipGetDevice(&dev));
+    HIPX(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    E->nwg = E->tune.persistent_wgs > 0 ? E->tune.persistent_wgs : 4 * std::max(ncu, 1);
+    E->uploaded = true;
+    return 0;
+}
+
+int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (nsteps < 0) return fail(GCMC_E_ARG, "negative step count");
+    if (nsteps == 0) return 0;
+    HIPX(hipEventRecord(E->ev0, E->stream));
+    if (mode == GCMC_MODE_PLAIN) {
+        if (E->staged) launch_plain<true>(E, nsteps);
+        else launch_plain<false>(E, nsteps);
+    } else if (mode == GCMC_MODE_BATCHED) {
+        if (E->staged) launch_batched<true>(E, nsteps);
+        else launch_batched<false>(E, nsteps);
+    } else if (mode == GCMC_MODE_GRAPH) {
+        if (!E->gexec || E->graph_nsteps != nsteps) {
+            if (E->gexec) {
+                hipGraphExecDestroy(E->gexec);
+                hipGraphDestroy(E->graph);
+                E->gexec = nullptr;
+            }
+            HIPX(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+            if (E->staged) launch_batched<true>(E, nsteps);
+            else launch_batched<false>(E, nsteps);
+            HIPX(hipStreamEndCapture(E->stream, &E->graph));
+            HIPX(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
+            E->graph_nsteps = nsteps;
+        }
+        HIPX(hipGraphLaunch(E->gexec, E->stream));
+    } else {
+        return fail(GCMC_E_ARG, "unknown run mode");
+    }
+    HIPX(hipGetLastError());
+    HIPX(hipEventRecord(E->ev1, E->stream));
+    HIPX(hipEventSynchronize(E->ev1));
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+    E->last_ms = ms;
+    std::vector<State> st;
+    int rc = read_states(E, st);
+    if (rc) return rc;
+    Sched sch{};
+    HIPX(hipMemcpy(&sch, E->d_sched, sizeof(sch), hipMemcpyDeviceToHost));
+    if (sch.abort) return fail(GCMC_E_DEVICE, "persistent kernel stopped at its spin limit");
+    for (size_t r = 0; r < st.size(); ++r) {
+        E->reps[r]->host_step = st[r].step;
+        if (st[r].step != st[r].end)
+            return fail(GCMC_E_DEVICE, "replica " + std::to_string(r) + " stopped at step " +
+                                           std::to_string(st[r].step) + " before its end " +
+                                           std::to_string(st[r].end));
+    }
+    return 0;
+}
+
+int gcmc_engine_synchronize(gcmc_engine *E)
+{
+    if (!E) return fail(GCMC_E_ARG, "null engine");
+    HIPX(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+int gcmc_engine_step(gcmc_engine *E, int r)
+{
+    if (!E || r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    return E->reps[r]->host_step;
+}
+
+int gcmc_engine_last_run_ms(gcmc_engine *E, double *ms)
+{
+    if (!E || !ms) return fail(GCMC_E_ARG, "null argument");
+    *ms = E->last_ms;
+    return 0;
+}
+
+int gcmc_engine_download(gcmc_engine *E, int r, AtomArray *frag, residue *res, Atom *atoms)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HostRep &H = *E->reps[r];
+    const InfoStruct &I = H.info;
+    HIPX(hipStreamSynchronize(E->stream));
+    State S;
+    HIPX(hipMemcpy(&S, E->d_state + r, sizeof(S), hipMemcpyDeviceToHost));
+    std::vector<float4> res_pos(I.totalResNum), atom(I.totalAtomNum);
+    std::vector<int> atype(I.totalAtomNum);
+    HIPX(hipMemcpy(res_pos.data(), H.rep.res_pos, res_pos.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(atom.data(), H.rep.atom, atom.size() * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(atype.data(), H.rep.atype, atype.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (frag)
+        for (int s = 0; s < I.fragTypeNum; ++s) frag[s].totalNum = S.totalNum[s];
+    if (res)
+        for (int g = 0; g < I.totalResNum; ++g) {
+            res[g].position[0] = res_pos[g].x;
+            res[g].position[1] = res_pos[g].y;
+            res[g].position[2] = res_pos[g].z;
+        }
+    if (atoms)
+        for (int a = 0; a < I.totalAtomNum; ++a) {
+            atoms[a].position[0] = atom[a].x;
+            atoms[a].position[1] = atom[a].y;
+            atoms[a].position[2] = atom[a].z;
+            atoms[a].charge = atom[a].w;
+            atoms[a].type = atype[a];
+        }
+    return 0;
+}
+
+int gcmc_engine_stats(gcmc_engine *E, int r, double *energy, long long *attempts, long long *accepts)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HIPX(hipStreamSynchronize(E->stream));
+    State S;
+    HIPX(hipMemcpy(&S, E->d_state + r, sizeof(S), hipMemcpyDeviceToHost));
+    const int T = E->reps[r]->info.fragTypeNum;
+    if (energy) *energy = S.energy;
+    if (attempts) std::memcpy(attempts, S.att, sizeof(long long) * 4 * T);
+    if (accepts) std::memcpy(accepts, S.acc, sizeof(long long) * 4 * T);
+    return 0;
+}
+
+int gcmc_engine_set_trace(gcmc_engine *E, int r, int max_steps, int max_k)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    if (max_steps < 0 || max_k < 0) return fail(GCMC_E_ARG, "negative trace size");
+    HostRep &H = *E->reps[r];
+    HIPX(hipStreamSynchronize(E->stream));
+    H.trbuf.release();
+    DevTrace &T = H.rep.tr;
+    T = DevTrace{};
+    if (max_steps > 0 && max_k > 0) {
+        const size_t S = max_steps, K = max_k;
+        T.max_steps = max_steps;
+        T.max_k = max_k;
+        T.first = H.host_step;
+        DevBuf &B = H.trbuf;
+        HIPX(B.alloc(&T.k, S));
+        HIPX(B.alloc(&T.ngroups, S));
+        HIPX(B.alloc(&T.naccept, S));
+        HIPX(B.alloc(&T.n_after, S));
+        HIPX(B.alloc(&T.etot, S));
+        HIPX(B.alloc(&T.centre, S * K * 3));
+        HIPX(B.alloc(&T.e, S * K * 2));
+        HIPX(B.alloc(&T.ninc, S * K * 2));
+        HIPX(B.alloc(&T.inchash, S * K * 2));
+        HIPX(B.alloc(&T.posehash, S * K));
+        HIPX(B.alloc(&T.idx, S * K));
+        HIPX(B.alloc(&T.gpick, S * K));
+        HIPX(B.alloc(&T.gacc, S * K));
+        HIPX(hipMemset(T.k, 0, S * sizeof(int)));
+        HIPX(hipMemset(T.ngroups, 0, S * sizeof(int)));
+        HIPX(hipMemset(T.naccept, 0, S * sizeof(int)));
+        HIPX(hipMemset(T.n_after, 0, S * sizeof(int)));
+        HIPX(hipMemset(T.etot, 0, S * sizeof(double)));
+        HIPX(hipMemset(T.centre, 0, S * K * 3 * sizeof(float)));
+        HIPX(hipMemset(T.e, 0, S * K * 2 * sizeof(double)));
+        HIPX(hipMemset(T.ninc, 0, S * K * 2 * sizeof(int)));
+        HIPX(hipMemset(T.inchash, 0, S * K * 2 * sizeof(uint32_t)));
+        HIPX(hipMemset(T.posehash, 0, S * K * sizeof(uint32_t)));
+        HIPX(hipMemset(T.idx, 0xff, S * K * sizeof(int)));
+        HIPX(hipMemset(T.gpick, 0xff, S * K * sizeof(int)));
+        HIPX(hipMemset(T.gacc, 0, S * K * sizeof(int)));
+    }
+    return sync_rep_struct(E, r);
+}
+
+int gcmc_engine_get_trace(gcmc_engine *E, int r, gcmc_trace *out)
+{
+    if (!E || !E->uploaded || !out) return fail(GCMC_E_STATE, "engine not uploaded or null trace");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    const DevTrace &T = E->reps[r]->rep.tr;
+    if (!T.k) return fail(GCMC_E_STATE, "tracing is off for this replica");
+    if (out->max_steps != T.max_steps || out->max_k != T.max_k)
+        return fail(GCMC_E_ARG, "trace shape differs from gcmc_engine_set_trace");
+    HIPX(hipStreamSynchronize(E->stream));
+    const size_t S = T.max_steps, K = T.max_k;
+    HIPX(hipMemcpy(out->k, T.k, S * sizeof(int), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->ngroups, T.ngroups, S * sizeof(int), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->naccept, T.naccept, S * sizeof(int), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->n_after, T.n_after, S * sizeof(int), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->etot, T.etot, S * sizeof(double), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->centre, T.centre, S * K * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->e, T.e, S * K * 2 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->ninc, T.ninc, S * K * 2 * sizeof(int), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->inchash, T.inchash, S * K * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->posehash, T.posehash, S * K * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->idx, T.idx, S * K * sizeof(int), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->gpick, T.gpick, S * K * sizeof(int), hipMemcpyDeviceToHost));
+    HIPX(hipMemcpy(out->gacc, T.gacc, S * K * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int gcmc_engine_eval_poses(gcmc_engine *E, int r, int species, int npose, const float *centres,
+                           const float *poses, const int *exclude, double *energy, int *ninc,
+                           uint32_t *inchash)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HostRep &H = *E->reps[r];
+    if (species < 0 || species >= H.info.fragTypeNum) return fail(GCMC_E_ARG, "bad species");
+    if (npose < 0) return fail(GCMC_E_ARG, "negative pose count");
+    if (npose == 0) return 0;
+    if (!centres || !poses || !energy) return fail(GCMC_E_ARG, "null array");
+    const int nf = H.frag[species].num_atoms;
+    std::vector<int> ex(npose, -1);
+    if (exclude) ex.assign(exclude, exclude + npose);
+    DevBuf B;
+    float *dc = nullptr, *dp = nullptr;
+    int *dx = nullptr, *dn = nullptr;
+    double *de = nullptr;
+    uint32_t *dh = nullptr;
+    hipError_t e = put(&dc, B, centres, 3 * (size_t)npose);
+    if (e == hipSuccess) e = put(&dp, B, poses, 3 * (size_t)npose * nf);
+    if (e == hipSuccess) e = put(&dx, B, ex.data(), ex.size());
+    if (e == hipSuccess) e = B.alloc(&de, npose);
+    if (e == hipSuccess) e = B.alloc(&dn, npose);
+    if (e == hipSuccess) e = B.alloc(&dh, npose);
+    if (e == hipSuccess) {
+        if (E->staged)
+            hipLaunchKernelGGL(k_eval_poses<true>, dim3(npose), dim3(WG), E->lay.total, E->stream, E->d_rep, r,
+                               species, dc, dp, dx, de, dn, dh, E->lay);
+        else
+            hipLaunchKernelGGL(k_eval_poses<false>, dim3(npose), dim3(WG), E->lay.total, E->stream, E->d_rep,
+                               r, species, dc, dp, dx, de, dn, dh, E->lay);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+    if (e == hipSuccess) e = hipMemcpy(energy, de, npose * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && ninc) e = hipMemcpy(ninc, dn, npose * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && inchash) e = hipMemcpy(inchash, dh, npose * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    B.release();
+    if (e != hipSuccess) return fail(GCMC_E_HIP, std::string("eval_poses: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int gcmc_host_within(int n, const float *a, const float *b, const float *cryst, float cutoff,
+                     unsigned char *out)
+{
+    if (n < 0 || (n > 0 && (!a || !b || !out)) || !cryst) return fail(GCMC_E_ARG, "bad arguments");
+    AxisCut ax[3];
+    for (int i = 0; i < 3; ++i) {
+        ax[i].L = cryst[i];
+        ax[i].t[0] = shift_threshold(cryst[i], 1);
+        ax[i].t[1] = shift_threshold(cryst[i], 2);
+        ax[i].t[2] = shift_threshold(cryst[i], 3);
+    }
+    const float r2max = r2_threshold(cutoff);
+    for (int q = 0; q < n; ++q)
+        out[q] = within(a[3 * q], a[3 * q + 1], a[3 * q + 2], b[3 * q], b[3 * q + 1], b[3 * q + 2], ax, r2max) ? 1 : 0;
+    return 0;
+}
+
+int runGCMC_hip(const InfoStruct *info, AtomArray *fragmentInfo, residue *residueInfo, Atom *atomInfo,
+                const float *grid, const float *ff, const int *moveArray)
+{
+    if (!info) return fail(GCMC_E_ARG, "null info");
+    gcmc_engine *E = nullptr;
+    gcmc_tuning t{};
+    t.pair_lds = 1;
+    int rc = gcmc_engine_create(&E, &t);
+    if (rc) return rc;
+    rc = gcmc_engine_add_system(E, info, fragmentInfo, residueInfo, atomInfo, grid, ff, moveArray,
+                                info->mcsteps);
+    if (rc >= 0) rc = gcmc_engine_upload(E);
+    if (rc >= 0) rc = gcmc_engine_run(E, info->mcsteps, GCMC_MODE_BATCHED);
+    if (rc >= 0) rc = gcmc_engine_download(E, 0, fragmentInfo, residueInfo, atomInfo);
+    std::string keep = g_err;
+    gcmc_engine_destroy(E);
+    g_err = keep;
+    return rc < 0 ? rc : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,959 @@
+// engine_kernels.h -- device phases of one GCMC move and the kernels that
+// schedule them.
+//
+// A move of one replica is three phases, each run by one 256-thread
+// workgroup:
+//   prep    decode moveArray[step]; pick slots (partial Fisher-Yates);
+//           trial centres                      (gcmc_move_*.h host side)
+//   item    one trial evaluation: build the pose, scan the cell list,
+//           group test, pair energies          (Gmove_*, calcEnergy)
+//   accept  grouping of trials, selection, Metropolis in FP64, commit
+//           (move_add/del/trn/rot host logic, Gupdate*)
+// The plain form launches item and accept kernels per move; the batched form
+// runs every replica in one persistent kernel where workgroups claim items
+// from any replica and the workgroup finishing a replica's last item runs its
+// accept and prep phases (no grid-wide barrier, replicas advance
+// independently).
+#pragma once
+
+#include "engine_types.h"
+
+namespace gcmc {
+
+struct Lds {
+    int *scell, *sfix, *soff, *incg, *incs, *incn;
+    float4 *frag;
+    int *frow;
+    float2 *pair;
+    int *misc;
+    double *dred;
+    int *members;
+    double *w;
+    int2 *accl;
+    int *J;
+};
+
+__device__ inline Lds carve(char *b, const LdsLayout &l)
+{
+    Lds s;
+    s.scell = (int *)(b + l.scell);
+    s.sfix = (int *)(b + l.sfix);
+    s.soff = (int *)(b + l.soff);
+    s.incg = (int *)(b + l.incg);
+    s.incs = (int *)(b + l.incs);
+    s.incn = (int *)(b + l.incn);
+    s.frag = (float4 *)(b + l.frag);
+    s.frow = (int *)(b + l.frow);
+    s.pair = (float2 *)(b + (l.pair >= 0 ? l.pair : 0));
+    s.misc = (int *)(b + l.misc);
+    s.dred = (double *)(b + l.dred);
+    s.members = (int *)(b + l.members);
+    s.w = (double *)(b + l.w);
+    s.accl = (int2 *)(b + l.accl);
+    s.J = (int *)(b + l.J);
+    return s;
+}
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// exclusive scan of a[0..n) in LDS, a[n] = total; returns total
+__device__ inline int wg_scan(int *a, int n, int *misc)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (n + WG - 1) / WG;
+    const int b = tid * per;
+    const int e = min(n, b + per);
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += a[i];
+    int x = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) misc[wave] = x;
+    __syncthreads();
+    if (tid == 0) {
+        int t = 0;
+        for (int w = 0; w < NWAVE; ++w) {
+            int v = misc[w];
+            misc[w] = t;
+            t += v;
+        }
+        misc[NWAVE] = t;
+    }
+    __syncthreads();
+    int base = misc[wave] + x - sum;
+    for (int i = b; i < e; ++i) {
+        int v = a[i];
+        a[i] = base;
+        base += v;
+    }
+    const int total = misc[NWAVE];
+    if (tid == 0) a[n] = total;
+    __syncthreads();
+    return total;
+}
+
+// largest e in [0, n) with a[e] <= m (a[0] = 0 <= m < a[n])
+__device__ inline int bsearch_le(const int *a, int n, int m)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a[mid] <= m) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---- pair energies of the buffered included groups ------------------------
+// lane <-> environment atom (float4 load, coalesced within a group), loop
+// over the fragment atoms held in LDS; FP32 terms, TwoSum into (acc, cmp).
+template <bool STAGED>
+__device__ inline void pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
+{
+    const int A = wg_scan(L.incn, n, L.misc);
+    const float2 *PT = STAGED ? L.pair : C.pair;
+    const float Lx = C.L[0], Ly = C.L[1], Lz = C.L[2];
+    const float ix = C.invL[0], iy = C.invL[1], iz = C.invL[2];
+    for (int t = threadIdx.x; t < A; t += WG) {
+        const int g = bsearch_le(L.incn, n, t);
+        const int a = L.incs[g] + (t - L.incn[g]);
+        const float4 at = C.atom[a];
+        const int ty = C.atype[a];
+        for (int f = 0; f < nf; ++f) {
+            const float4 fr = L.frag[f];
+            const float2 pp = PT[L.frow[f] + ty];
+            float dx = at.x - fr.x, dy = at.y - fr.y, dz = at.z - fr.z;
+            dx = fmaf(-rintf(dx * ix), Lx, dx);
+            dy = fmaf(-rintf(dy * iy), Ly, dy);
+            dz = fmaf(-rintf(dz * iz), Lz, dz);
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float ir = __builtin_amdgcn_rsqf(r2);
+            ir = ir * fmaf(-0.5f * r2 * ir, ir, 1.5f);     // one Newton step
+            const float ir2 = ir * ir;
+            const float s2 = pp.x * ir2;
+            const float s6 = s2 * s2 * s2;
+            const float lj = pp.y * fmaf(s6, s6, -s6);     // 4 eps (s^12 - s^6)
+            const float el = (fr.w * at.w) * ir;            // K q_f q_a / r
+            const float term = lj + el;
+            const float s = acc + term;                     // TwoSum
+            const float bp = s - acc;
+            const float err = (acc - (s - bp)) + (term - bp);
+            acc = s;
+            cmp += err;
+        }
+    }
+    __syncthreads();
+}
+
+// ---- calcEnergy over the cell list -----------------------------------------
+// Candidates: fixed and mobile groups of the stencil cells around the test
+// centre plus the overflow bucket; kept when gid != exclude and the FP32
+// distanceP test passes (gcmc_energy.h:152-191).  Thread 0 receives results.
+template <bool STAGED>
+__device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, int nf, int exclude,
+                                   const Lds &L, double *outE, int *outN, uint32_t *outH)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int cc = cell_index(C.L, C.start, C.inv_h, C.nc, cx, cy, cz);
+    const int c2 = cc % C.nc[2];
+    const int c1 = (cc / C.nc[2]) % C.nc[1];
+    const int c0 = cc / (C.nc[1] * C.nc[2]);
+    const int ne = C.nst + 1;
+    for (int e = tid; e < ne; e += WG) {
+        int cell, fc, mc;
+        if (e < C.nst) {
+            const int code = C.stencil[e];
+            int i0 = c0 + (code & 255) - 128;
+            int i1 = c1 + ((code >> 8) & 255) - 128;
+            int i2 = c2 + ((code >> 16) & 255) - 128;
+            i0 = ((i0 % C.nc[0]) + C.nc[0]) % C.nc[0];
+            i1 = ((i1 % C.nc[1]) + C.nc[1]) % C.nc[1];
+            i2 = ((i2 % C.nc[2]) + C.nc[2]) % C.nc[2];
+            cell = (i0 * C.nc[1] + i1) * C.nc[2] + i2;
+            fc = C.fix_start[cell + 1] - C.fix_start[cell];
+            mc = C.mob_cnt[cell];
+        } else {
+            cell = C.ncell;
+            fc = 0;
+            mc = C.mob_cnt[C.ncell];
+        }
+        L.scell[e] = cell;
+        L.sfix[e] = fc;
+        L.soff[e] = fc + mc;
+    }
+    __syncthreads();
+    const int M = wg_scan(L.soff, ne, L.misc);
+    float acc = 0.0f, cmp = 0.0f;
+    int nin = 0;
+    uint32_t hs = 0;
+    if (tid == 0) L.misc[16] = 0;
+    __syncthreads();
+    for (int base = 0; base < M; base += WG) {
+        const int m = base + tid;
+        bool inc = false;
+        int gid = -1;
+        int2 rr = make_int2(0, 0);
+        if (m < M) {
+            const int e = bsearch_le(L.soff, ne, m);
+            const int local = m - L.soff[e];
+            const int cell = L.scell[e];
+            const int fc = L.sfix[e];
+            float4 p;
+            if (local < fc) {
+                p = C.fix_rec[C.fix_start[cell] + local];
+                gid = __float_as_int(p.w);
+            } else {
+                const int idx = (cell < C.ncell ? cell * C.cap : C.ovf_base) + (local - fc);
+                gid = C.mob_slot[idx];
+                p = C.res_pos[gid];
+            }
+            inc = (gid != exclude) && within(cx, cy, cz, p.x, p.y, p.z, C.ax, C.r2max);
+            if (inc) {
+                rr = C.res_rng[gid];
+                nin += 1;
+                hs += group_hash((uint32_t)gid);
+            }
+        }
+        const unsigned long long bal = __ballot(inc);
+        const int loff = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) L.misc[8 + wave] = __popcll(bal);
+        __syncthreads();
+        int wbase = L.misc[16];
+        for (int w = 0; w < wave; ++w) wbase += L.misc[8 + w];
+        if (inc) {
+            const int pos = wbase + loff;
+            L.incg[pos] = gid;
+            L.incs[pos] = rr.x;
+            L.incn[pos] = rr.y;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int t = L.misc[16];
+            for (int w = 0; w < NWAVE; ++w) t += L.misc[8 + w];
+            L.misc[16] = t;
+        }
+        __syncthreads();
+        const int ninc = L.misc[16];
+        if (ninc > INC_CAP - WG || base + WG >= M) {
+            if (ninc > 0) pair_pass<STAGED>(C, L, ninc, nf, acc, cmp);
+            if (tid == 0) L.misc[16] = 0;
+            __syncthreads();
+        }
+    }
+    double v = (double)acc + (double)cmp;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_down(v, off, 64);
+        nin += __shfl_down(nin, off, 64);
+        hs += __shfl_down(hs, off, 64);
+    }
+    if (lane == 0) {
+        L.dred[wave] = v;
+        L.misc[20 + wave] = nin;
+        L.misc[24 + wave] = (int)hs;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double E = 0.0;
+        int N = 0;
+        uint32_t H = 0;
+        for (int w = 0; w < NWAVE; ++w) {
+            E += L.dred[w];
+            N += L.misc[20 + w];
+            H += (uint32_t)L.misc[24 + w];
+        }
+        *outE = E;
+        *outN = N;
+        *outH = H;
+    }
+    __syncthreads();
+}
+
+__device__ inline uint32_t pose_hash(const float4 *p, int nf)
+{
+    uint32_t h = 2166136261u;
+    for (int a = 0; a < nf; ++a) {
+        h = (h ^ __float_as_uint(p[a].x)) * 16777619u;
+        h = (h ^ __float_as_uint(p[a].y)) * 16777619u;
+        h = (h ^ __float_as_uint(p[a].z)) * 16777619u;
+    }
+    return h;
+}
+
+__device__ inline bool tracing(const Rep &C, int step, int &row)
+{
+    row = step - C.tr.first;
+    return C.tr.k != nullptr && row >= 0 && row < C.tr.max_steps;
+}
+
+// ---- one trial evaluation ----------------------------------------------------
+// item < k: trial item (insert/delete) or E_old (translate/rotate);
+// item >= k: E_new of trial item-k (translate/rotate).
+template <bool STAGED>
+__device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
+    const Species &F = C.sp[spi];
+    const int nf = F.nf;
+    int t = item, phase = 0;
+    if (kind >= K_TRN && item >= k) {
+        t = item - k;
+        phase = 1;
+    }
+    if (STAGED)
+        for (int i = tid; i < C.X * C.Y; i += WG) L.pair[i] = C.pair[i];
+    const float4 tcv = C.tc[t];
+    const int slot = (kind == K_INS) ? -1 : C.tslot[t];
+    const int gid = (kind == K_INS) ? -1 : F.startRes + slot;
+    const float4 cold = (gid >= 0) ? C.res_pos[gid] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float cx, cy, cz;
+    if (kind == K_INS || phase == 1 || kind == K_DEL) {
+        cx = tcv.x; cy = tcv.y; cz = tcv.z;
+    } else {
+        cx = cold.x; cy = cold.y; cz = cold.z;
+    }
+    const bool newpose = (kind == K_INS) || (kind == K_DEL) || (phase == 1);
+    if (tid < nf) {
+        const int a = tid;
+        const float4 tm = F.tmpl[a];
+        float px, py, pz;
+        if (kind == K_INS) {
+            const u32x4 y = draw(C.seed, (uint32_t)step, (uint32_t)t, P_ROTATION);
+            float R[9], v[3];
+            rotation_matrix(u01f(y.x), u01f(y.y), u01f(y.z), u01f(y.w) * 0x1.921fb6p+2f, R);
+            rotate(R, tm.x, tm.y, tm.z, v);
+            px = v[0] + cx; py = v[1] + cy; pz = v[2] + cz;
+        } else {
+            const int2 rr = C.res_rng[gid];
+            const float4 cur = C.atom[rr.x + a];
+            if (kind == K_DEL || phase == 0) {
+                px = cur.x; py = cur.y; pz = cur.z;
+            } else if (kind == K_TRN) {
+                const u32x4 x = draw(C.seed, (uint32_t)step, (uint32_t)t, P_JITTER);
+                const float u[3] = {u01f(x.x), u01f(x.y), u01f(x.z)};
+                const float co[3] = {cold.x, cold.y, cold.z};
+                float r[3];
+                for (int i = 0; i < 3; ++i) {
+                    float d = u[i] * C.dx;
+                    if (co[i] + d > C.L[i] + C.start[i]) d = d - C.L[i];
+                    if (co[i] + d < C.start[i]) d = d + C.L[i];
+                    r[i] = d;
+                }
+                px = cur.x + r[0]; py = cur.y + r[1]; pz = cur.z + r[2];
+            } else {
+                float cen0 = 0.f, cen1 = 0.f, cen2 = 0.f;
+                const float fn = (float)nf;
+                for (int b = 0; b < nf; ++b) {
+                    const float4 q = C.atom[rr.x + b];
+                    cen0 = cen0 + q.x / fn;
+                    cen1 = cen1 + q.y / fn;
+                    cen2 = cen2 + q.z / fn;
+                }
+                const u32x4 y = draw(C.seed, (uint32_t)step, (uint32_t)t, P_ROTATION);
+                float R[9], v[3];
+                rotation_matrix(u01f(y.x), u01f(y.y), u01f(y.z), u01f(y.w) * 0x1.921fb6p+2f, R);
+                rotate(R, cur.x - cen0, cur.y - cen1, cur.z - cen2, v);
+                px = v[0] + cen0; py = v[1] + cen1; pz = v[2] + cen2;
+            }
+        }
+        L.frag[a] = make_float4(px, py, pz, F.kq[a]);
+        L.frow[a] = F.trow[a];
+        if (kind == K_INS || phase == 1) C.pose[t * MAXF + a] = make_float4(px, py, pz, 0.f);
+    }
+    __syncthreads();
+
+    // adjacency row of trial t over the trial centres (grouping of the accept phase)
+    if (kind != K_DEL && phase == 0) {
+        for (int j0 = 0; j0 < k; j0 += WG) {
+            const int j = j0 + tid;
+            bool b = false;
+            if (j < k && j != t) {
+                const float4 cj = C.tc[j];
+                b = within(tcv.x, tcv.y, tcv.z, cj.x, cj.y, cj.z, C.ax, C.r2max);
+            }
+            const unsigned long long bal = __ballot(b);
+            if (lane == 0) {
+                const int wi = (j0 + wave * 64) >> 5;
+                if (wi < ADJW) C.adj[t * ADJW + wi] = (uint32_t)bal;
+                if (wi + 1 < ADJW) C.adj[t * ADJW + wi + 1] = (uint32_t)(bal >> 32);
+            }
+        }
+    }
+
+    double E;
+    int nin;
+    uint32_t hh;
+    eval_energy<STAGED>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh);
+    if (tid == 0) {
+        C.E[phase * k + t] = E;
+        int row;
+        if (tracing(C, step, row) && t < C.tr.max_k) {
+            const int q = row * C.tr.max_k + t;
+            C.tr.e[2 * q + phase] = E;
+            C.tr.ninc[2 * q + phase] = nin;
+            C.tr.inchash[2 * q + phase] = hh;
+            if (newpose) {
+                C.tr.posehash[q] = pose_hash(L.frag, nf);
+                C.tr.centre[3 * q + 0] = tcv.x;
+                C.tr.centre[3 * q + 1] = tcv.y;
+                C.tr.centre[3 * q + 2] = tcv.z;
+                C.tr.idx[q] = slot;
+            }
+        }
+    }
+}
+
+// ---- cell-list maintenance (single thread) ----------------------------------
+__device__ inline void cell_insert(const Rep &C, int gid, float x, float y, float z)
+{
+    const int c = cell_index(C.L, C.start, C.inv_h, C.nc, x, y, z);
+    const int n = C.mob_cnt[c];
+    int pos;
+    if (n < C.cap) {
+        pos = c * C.cap + n;
+        C.mob_cnt[c] = n + 1;
+    } else {
+        const int o = C.mob_cnt[C.ncell];
+        pos = C.ovf_base + o;
+        C.mob_cnt[C.ncell] = o + 1;
+    }
+    C.mob_slot[pos] = gid;
+    C.mob_loc[gid] = pos;
+}
+
+__device__ inline void cell_remove(const Rep &C, int gid)
+{
+    const int pos = C.mob_loc[gid];
+    int c, base;
+    if (pos >= C.ovf_base) {
+        c = C.ncell;
+        base = C.ovf_base;
+    } else {
+        c = pos / C.cap;
+        base = c * C.cap;
+    }
+    const int n = C.mob_cnt[c] - 1;
+    const int moved = C.mob_slot[base + n];
+    C.mob_slot[pos] = moved;
+    C.mob_loc[moved] = pos;
+    C.mob_cnt[c] = n;
+    C.mob_loc[gid] = -1;
+}
+
+__device__ inline void cell_rename(const Rep &C, int from, int to)
+{
+    const int pos = C.mob_loc[from];
+    C.mob_slot[pos] = to;
+    C.mob_loc[to] = pos;
+    C.mob_loc[from] = -1;
+}
+
+__device__ inline double trial_value(const Rep &C, int kind, int k, int t)
+{
+    return kind == K_INS ? C.E[t] : C.E[k + t] - C.E[t];
+}
+
+// ---- selection and Metropolis (one wave) --------------------------------------
+// Greedy grouping in ascending trial order: leader = first unused trial,
+// members = unused trials within the cutoff of the leader; Boltzmann weights
+// relative to the group minimum; cumulative pick; acceptance
+//   insert:  min(1, cav/(m P)/(N+1) exp(B - beta E))   (gcmc_move_add.h:164-174)
+//   move:    min(1, 1/(m P) exp(-beta dE))             (gcmc_move_trn.h:215-222)
+// On accept the unused trials within the cutoff of the pick are dropped.
+__device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int row, bool trace)
+{
+    const int lane = threadIdx.x & 63;
+    const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
+    const Species &F = C.sp[spi];
+    const bool ins = kind == K_INS;
+    const int nw = (k + 31) >> 5;
+    uint32_t u = 0u;
+    if (lane < nw) {
+        const int hi = k - lane * 32;
+        u = hi >= 32 ? 0xffffffffu : ((1u << hi) - 1u);
+    }
+    int N = S.totalNum[spi];
+    const int maxN = F.maxNum;
+    int g = 0, na = 0;
+    double et = S.energy;
+    while (true) {
+        const unsigned long long bal = __ballot(u != 0u);
+        if (bal == 0ull) break;
+        const int f = __ffsll((long long)bal) - 1;
+        const uint32_t wf = __shfl(u, f, 64);
+        const int it = f * 32 + __ffs(wf) - 1;
+        uint32_t m = (lane < nw) ? (C.adj[it * ADJW + lane] & u) : 0u;
+        if (lane == (it >> 5)) m |= 1u << (it & 31);
+        u &= ~m;
+        const int cnt = __popc(m);
+        int x = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        const int mtot = __shfl(x, 63, 64);
+        int o = x - cnt;
+        uint32_t mm = m;
+        while (mm) {
+            const int b = __ffs(mm) - 1;
+            mm &= mm - 1u;
+            L.members[o++] = lane * 32 + b;
+        }
+        wave_sync();
+        double emin = __builtin_huge_val();
+        for (int q = lane; q < mtot; q += 64) {
+            const double v = trial_value(C, kind, k, L.members[q]);
+            emin = v < emin ? v : emin;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o2 = __shfl_xor(emin, off, 64);
+            emin = o2 < emin ? o2 : emin;
+        }
+        for (int q = lane; q < mtot; q += 64)
+            L.w[q] = exp(-BETA * (trial_value(C, kind, k, L.members[q]) - emin));
+        wave_sync();
+        double sumP = 0.0;
+        for (int q = 0; q < mtot; ++q) sumP += L.w[q];
+        const u32x4 xr = draw(C.seed, (uint32_t)step, (uint32_t)g, P_DECIDE);
+        const double ran_sel = u01d(xr.x, xr.y);
+        const double ran_acc = u01d(xr.z, xr.w);
+        int pick = it, pq = 0;
+        if (sumP != 0.0) {
+            double cum = 0.0;
+            pick = L.members[mtot - 1];
+            pq = mtot - 1;
+            for (int q = 0; q < mtot; ++q) {
+                cum += L.w[q] / sumP;
+                if (ran_sel < cum) {
+                    pick = L.members[q];
+                    pq = q;
+                    break;
+                }
+            }
+        }
+        const double P = sumP != 0.0 ? L.w[pq] / sumP : 1.0;
+        const double vp = trial_value(C, kind, k, pick);
+        double pp;
+        if (ins)
+            pp = ((double)C.cav / ((double)mtot * P)) / ((double)N + 1.0) * exp(F.B - BETA * vp);
+        else
+            pp = (1.0 / ((double)mtot * P)) * exp(-BETA * vp);
+        const double p = pp < 1.0 ? pp : 1.0;
+        const bool acc = ran_acc < p;
+        int rec = acc ? 1 : 0;
+        if (acc) {
+            const uint32_t ap = (lane < nw) ? C.adj[pick * ADJW + lane] : 0u;
+            u &= ~ap;
+            if (ins) {
+                if (N < maxN) {
+                    if (lane == 0) L.accl[na] = make_int2(pick, N);
+                    N += 1;
+                    na += 1;
+                    et += vp;
+                } else {
+                    rec = 2;
+                }
+            } else {
+                if (lane == 0) L.accl[na] = make_int2(pick, C.tslot[pick]);
+                na += 1;
+                et += vp;
+            }
+        }
+        if (trace && lane == 0 && g < C.tr.max_k) {
+            C.tr.gpick[row * C.tr.max_k + g] = pick;
+            C.tr.gacc[row * C.tr.max_k + g] = rec;
+        }
+        g += 1;
+        wave_sync();
+    }
+    if (lane == 0) {
+        L.misc[1] = na;
+        L.misc[2] = g;
+        L.misc[3] = N;
+        S.energy = et;
+        S.acc[spi * 4 + kind] += na;
+    }
+}
+
+// ---- accept phase: decisions and commit ------------------------------------------
+__device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
+{
+    const int tid = threadIdx.x;
+    const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
+    const Species &F = C.sp[spi];
+    const int nf = F.nf;
+    int row;
+    const bool trace = tracing(C, step, row);
+    if (tid == 0) {
+        S.att[spi * 4 + kind] += 1;
+        L.misc[1] = 0;
+        L.misc[2] = 0;
+        L.misc[3] = S.totalNum[spi];
+    }
+    __syncthreads();
+    if (kind == K_DEL && k == 1) {
+        const int N = S.totalNum[spi];
+        if (tid == 0) {
+            const double E = C.E[0];
+            const u32x4 x = draw(C.seed, (uint32_t)step, 0u, P_DECIDE);
+            const double ran = u01d(x.z, x.w);
+            const double pp = (double)N * exp(-F.B - BETA * (-E));
+            const double p = pp < 1.0 ? pp : 1.0;
+            const int acc = ran < p;
+            L.misc[1] = acc;
+            L.misc[2] = 1;
+            if (trace) {
+                C.tr.gpick[row * C.tr.max_k] = 0;
+                C.tr.gacc[row * C.tr.max_k] = acc;
+            }
+            if (acc) {
+                S.energy -= E;
+                S.acc[spi * 4 + kind] += 1;
+            }
+        }
+        __syncthreads();
+        if (L.misc[1]) {
+            // GupdateDel (gcmc_move_del.h:55-94): swap-remove with the last slot
+            const int newN = N - 1;
+            const int hole = F.startRes + C.tslot[0], last = F.startRes + newN;
+            if (newN > 0 && hole != last) {
+                const int2 rh = C.res_rng[hole], rl = C.res_rng[last];
+                for (int a = tid; a < nf; a += WG) {
+                    const float4 src = C.atom[rl.x + a];
+                    float4 dst = C.atom[rh.x + a];
+                    dst.x = src.x; dst.y = src.y; dst.z = src.z;
+                    C.atom[rh.x + a] = dst;
+                }
+            }
+            if (tid == 0) {
+                cell_remove(C, hole);
+                if (newN > 0 && hole != last) {
+                    const float4 pl = C.res_pos[last];
+                    float4 ph = C.res_pos[hole];
+                    ph.x = pl.x; ph.y = pl.y; ph.z = pl.z;
+                    C.res_pos[hole] = ph;
+                    cell_rename(C, last, hole);
+                }
+                S.totalNum[spi] = newN;
+                L.misc[3] = newN;
+            }
+        }
+    } else if (k > 0 && kind != K_DEL) {
+        if ((tid >> 6) == 0) greedy_select(C, S, L, row, trace);
+        __syncthreads();
+        const int na = L.misc[1];
+        // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses
+        for (int idx = tid; idx < na * nf; idx += WG) {
+            const int q = idx / nf, a = idx - q * nf;
+            const int2 ts = L.accl[q];
+            const int gid = F.startRes + ts.y;
+            const int2 rr = C.res_rng[gid];
+            const float4 p = C.pose[ts.x * MAXF + a];
+            float4 dst = C.atom[rr.x + a];
+            dst.x = p.x; dst.y = p.y; dst.z = p.z;
+            if (kind == K_INS) {
+                dst.w = F.tmpl[a].w;
+                C.atype[rr.x + a] = F.ttype[a];
+            }
+            C.atom[rr.x + a] = dst;
+        }
+        if (tid == 0) {
+            for (int q = 0; q < na; ++q) {
+                const int2 ts = L.accl[q];
+                const int gid = F.startRes + ts.y;
+                const float4 c = C.tc[ts.x];
+                if (kind == K_INS) {
+                    C.res_pos[gid] = make_float4(c.x, c.y, c.z, 0.f);
+                    cell_insert(C, gid, c.x, c.y, c.z);
+                    if (trace && ts.x < C.tr.max_k) C.tr.idx[row * C.tr.max_k + ts.x] = ts.y;
+                } else if (kind == K_TRN) {
+                    const float4 old = C.res_pos[gid];
+                    C.res_pos[gid] = make_float4(c.x, c.y, c.z, old.w);
+                    const int c_old = cell_index(C.L, C.start, C.inv_h, C.nc, old.x, old.y, old.z);
+                    const int c_new = cell_index(C.L, C.start, C.inv_h, C.nc, c.x, c.y, c.z);
+                    const int pos = C.mob_loc[gid];
+                    const bool in_ovf = pos >= C.ovf_base;
+                    if (c_old != c_new || in_ovf) {
+                        cell_remove(C, gid);
+                        cell_insert(C, gid, c.x, c.y, c.z);
+                    }
+                }
+            }
+            if (kind == K_INS) S.totalNum[spi] = L.misc[3];
+        }
+    }
+    if (tid == 0) {
+        if (trace) {
+            C.tr.k[row] = k;
+            C.tr.ngroups[row] = L.misc[2];
+            C.tr.naccept[row] = L.misc[1];
+            C.tr.n_after[row] = S.totalNum[spi];
+            C.tr.etot[row] = S.energy;
+        }
+        S.step = step + 1;
+    }
+    __syncthreads();
+}
+
+// ---- prep phase: picks and trial centres of S.step ------------------------------
+__device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
+{
+    const int tid = threadIdx.x;
+    const int step = S.step;
+    const int mv = C.moves[step];
+    const int spi = mv >> 2, kind = mv & 3;
+    const Species &F = C.sp[spi];
+    const int N = S.totalNum[spi];
+    int k;
+    if (kind == K_INS) k = (N >= F.maxNum) ? 0 : F.conf;
+    else if (kind == K_DEL) k = (N == 0) ? 0 : 1;
+    else k = F.conf < N ? F.conf : N;
+    if (k > C.maxk) k = C.maxk;
+    if (kind != K_INS && k > 0) {
+        // partial Fisher-Yates (gcmc_move_trn.h:135-149): pick i takes the value at
+        // position J[i] after the swaps 0..i-1, found by tracing the swaps back
+        for (int i = tid; i < k; i += WG) {
+            const u32x4 x = draw(C.seed, (uint32_t)step, (uint32_t)i, P_PICK);
+            L.J[i] = i + (int)below(x.x, (uint32_t)(N - i));
+        }
+        __syncthreads();
+        for (int i = tid; i < k; i += WG) {
+            int p = L.J[i];
+            for (int m = i - 1; m >= 0; --m) {
+                const int jm = L.J[m];
+                p = (p == m) ? jm : ((p == jm) ? m : p);
+            }
+            C.tslot[i] = p;
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < k; t += WG) {
+        float c[3];
+        if (kind == K_INS) {
+            const u32x4 x = draw(C.seed, (uint32_t)step, (uint32_t)t, P_JITTER);
+            const uint32_t g = below(x.w, (uint32_t)C.G);
+            const float u[3] = {u01f(x.x), u01f(x.y), u01f(x.z)};
+            for (int i = 0; i < 3; ++i) c[i] = C.grid[3 * g + i] + u[i] * C.dx;
+        } else {
+            const float4 r = C.res_pos[F.startRes + C.tslot[t]];
+            c[0] = r.x; c[1] = r.y; c[2] = r.z;
+            if (kind == K_TRN) {
+                const u32x4 x = draw(C.seed, (uint32_t)step, (uint32_t)t, P_JITTER);
+                const float u[3] = {u01f(x.x), u01f(x.y), u01f(x.z)};
+                for (int i = 0; i < 3; ++i) {
+                    float d = u[i] * C.dx;
+                    if (c[i] + d > C.L[i] + C.start[i]) d = d - C.L[i];
+                    if (c[i] + d < C.start[i]) d = d + C.L[i];
+                    c[i] = c[i] + d;
+                }
+            }
+        }
+        C.tc[t] = make_float4(c[0], c[1], c[2], 0.f);
+    }
+    if (tid == 0) {
+        S.kind = kind;
+        S.sp = spi;
+        S.k = k;
+        S.nitems = (kind == K_TRN || kind == K_ROT) ? 2 * k : k;
+        S.done = 0;
+    }
+    __syncthreads();
+}
+
+// advance past empty moves and publish the next move with work (batched form)
+__device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L)
+{
+    while (true) {
+        if (S.step >= S.end) {
+            if (threadIdx.x == 0) {
+                __threadfence();
+                __hip_atomic_store(&S.claim, claim_word(S.step, 0, 0), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&sch->finished, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            break;
+        }
+        prep_step(C, S, L);
+        if (S.nitems > 0) {
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&S.claim, claim_word(S.step, S.nitems, 0), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        accept_step(C, S, L);
+    }
+    __syncthreads();
+}
+
+// ---- kernels ---------------------------------------------------------------------
+
+// start of a run: prepare every replica's current move
+// start of a run: set the end step of every replica and reset the scheduler
+__global__ void k_begin(State *st, const Rep *reps, int r0, int nrep, int nsteps, Sched *sch)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        sch->finished = 0;
+        sch->abort = 0;
+        sch->progress = 0;
+    }
+    if (i < nrep) {
+        State &S = st[r0 + i];
+        const long long e = (long long)S.step + nsteps;
+        S.end = (int)(e < reps[r0 + i].nmoves ? e : reps[r0 + i].nmoves);
+    }
+}
+
+// prepare the current move of replicas r0 + blockIdx.x (batched: publish it)
+template <bool STAGED>
+__global__ __launch_bounds__(WG) void k_start(const Rep *reps, State *st, Sched *sch, LdsLayout lay,
+                                             int r0, int batched)
+{
+    extern __shared__ __align__(16) char smem[];
+    const Lds L = carve(smem, lay);
+    const int r = r0 + blockIdx.x;
+    if (batched) advance_and_publish(reps[r], st[r], sch, L);
+    else if (st[r].step < st[r].end) prep_step(reps[r], st[r], L);
+}
+
+template <bool STAGED>
+__global__ __launch_bounds__(WG) void k_eval_plain(const Rep *reps, const State *st, int r, LdsLayout lay)
+{
+    extern __shared__ __align__(16) char smem[];
+    const State &S = st[r];
+    if (S.step >= S.end || (int)blockIdx.x >= S.nitems) return;
+    const Lds L = carve(smem, lay);
+    eval_item<STAGED>(reps[r], S, blockIdx.x, L);
+}
+
+template <bool STAGED>
+__global__ __launch_bounds__(WG) void k_accept_plain(const Rep *reps, State *st, int r, LdsLayout lay)
+{
+    extern __shared__ __align__(16) char smem[];
+    State &S = st[r];
+    if (S.step >= S.end) return;
+    const Lds L = carve(smem, lay);
+    accept_step(reps[r], S, L);
+    if (S.step < S.end) prep_step(reps[r], S, L);
+}
+
+// persistent form: workgroups claim evaluations from any replica; the
+// workgroup completing a replica's last evaluation runs its accept and prep
+// phases and publishes the next move.  Exits when every replica finished or
+// after spin_limit idle polls (then sch->abort is set).
+template <bool STAGED>
+__global__ __launch_bounds__(WG) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
+                                               LdsLayout lay, int spin_limit)
+{
+    extern __shared__ __align__(16) char smem[];
+    const Lds L = carve(smem, lay);
+    const int tid = threadIdx.x;
+    int idle = 0, seen = -1;
+    while (true) {
+        if (tid == 0) {
+            int found = -1, item = 0;
+            const int prog = __hip_atomic_load(&sch->progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prog != seen) {
+                seen = prog;
+                idle = 0;
+            }
+            if (__hip_atomic_load(&sch->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= R ||
+                __hip_atomic_load(&sch->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                found = -2;
+            } else {
+                for (int j = 0; j < R; ++j) {
+                    const int r = (int)((blockIdx.x + (unsigned)j) % (unsigned)R);
+                    const unsigned long long w =
+                        __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if ((w & 0xFFFFFull) >= ((w >> 20) & 0xFFFFFull)) continue;
+                    const unsigned long long old = __hip_atomic_fetch_add(
+                        &st[r].claim, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                    const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
+                    if (c < n) {
+                        found = r;
+                        item = c;
+                        break;
+                    }
+                }
+            }
+            if (found == -1 && ++idle > spin_limit) {
+                // no replica advanced during spin_limit polls: stop instead of hanging
+                __hip_atomic_store(&sch->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                found = -2;
+            }
+            L.misc[28] = found;
+            L.misc[29] = item;
+        }
+        __syncthreads();
+        const int found = L.misc[28];
+        const int item = L.misc[29];
+        __syncthreads();
+        if (found == -2) break;
+        if (found == -1) {
+            __builtin_amdgcn_s_sleep(8);
+            continue;
+        }
+        __threadfence();
+        const Rep &C = reps[found];
+        State &S = st[found];
+        eval_item<STAGED>(C, S, item, L);
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            L.misc[30] = (old == S.nitems - 1) ? 1 : 0;
+        }
+        __syncthreads();
+        if (L.misc[30]) {
+            __threadfence();
+            accept_step(C, S, L);
+            advance_and_publish(C, S, sch, L);
+            if (tid == 0) __hip_atomic_fetch_add(&sch->progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+}
+
+// energies of given poses (tests and the adjoint's forward)
+template <bool STAGED>
+__global__ __launch_bounds__(WG) void k_eval_poses(const Rep *reps, int r, int spi, const float *centres,
+                                                  const float *poses, const int *excl, double *E, int *ninc,
+                                                  uint32_t *hash, LdsLayout lay)
+{
+    extern __shared__ __align__(16) char smem[];
+    const Lds L = carve(smem, lay);
+    const Rep &C = reps[r];
+    const Species &F = C.sp[spi];
+    const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x;
+    if (STAGED)
+        for (int q = tid; q < C.X * C.Y; q += WG) L.pair[q] = C.pair[q];
+    if (tid < nf) {
+        const float *p = poses + ((size_t)i * nf + tid) * 3;
+        L.frag[tid] = make_float4(p[0], p[1], p[2], F.kq[tid]);
+        L.frow[tid] = F.trow[tid];
+    }
+    __syncthreads();
+    double e;
+    int n;
+    uint32_t h;
+    eval_energy<STAGED>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n, &h);
+    if (tid == 0) {
+        E[i] = e;
+        ninc[i] = n;
+        hash[i] = h;
+    }
+}
+
+}  // namespace gcmc

@@ -1,0 +1,139 @@
+// engine_types.h -- device-resident layout of the GCMC engine.
+//
+// One "replica" is one independent GCMC system (pyGCMC runs exactly one per
+// call, gcmc/cpp/gcmc.cu:21).  Its data lives in HBM as structure-of-arrays:
+//   res_pos  float4[R]   group centres (x, y, z, -)           gcmc.h residue
+//   res_rng  int2[R]     (atomStart, atomNum)
+//   atom     float4[A]   (x, y, z, charge)                     gcmc.h Atom
+//   atype    int[A]      LJ type (column of the pair table)
+//   pair     float2[X*Y] (100*sigma^2, 4*epsilon) per (fragment type, type)
+// plus a cell list over group centres: fixed groups in a static CSR
+// (fix_start / fix_rec sorted by cell), mobile groups (fragment slots) in
+// per-cell buckets of `cap` slots with one overflow bucket.
+#pragma once
+
+#include <stdint.h>
+
+#include "gcmc_math.h"
+
+namespace gcmc {
+
+constexpr int WG = 256;              // threads of every engine workgroup
+constexpr int NWAVE = WG / 64;
+constexpr int MAXT = 32;             // species per system
+constexpr int MAXF = 20;             // atoms per fragment (AtomArray.atoms[20])
+constexpr int MAXK = 2048;           // trials per move (confBias)
+constexpr int ADJW = MAXK / 32;      // words of one adjacency row
+constexpr int INC_CAP = 1024;        // included groups buffered before a pair pass
+constexpr double BETA = 1.0 / (0.0083115 * 300.0);  // gcmc/cpp/gcmc.h:40-41
+constexpr double KCOUL = 1388.431112;                // gcmc/cpp/gcmc_energy.h:137
+
+enum Kind { K_INS = 0, K_DEL = 1, K_TRN = 2, K_ROT = 3 };
+
+struct Species {
+    int startRes, maxNum, nf, conf;
+    int pad[4];
+    double B;              // beta*muex + ln(L0 L1 L2 conc 6.023e-4)
+    float4 tmpl[MAXF];     // centred template (x, y, z, charge)
+    float kq[MAXF];        // 1388.431112 * charge, rounded once
+    int trow[MAXF];        // type * Y: row of the pair table
+    int ttype[MAXF];       // type
+};
+
+struct DevTrace {
+    int max_steps, max_k, first;
+    int *k, *ngroups, *naccept, *n_after;
+    double *etot;
+    float *centre;
+    double *e;
+    int *ninc;
+    uint32_t *inchash, *posehash;
+    int *idx, *gpick, *gacc;
+};
+
+struct Rep {
+    AxisCut ax[3];
+    float L[3], start[3], invL[3];
+    float rc, r2max, dx, cav;
+    float inv_h[3];
+    int nc[3], ncell;
+    int G, T, X, Y;
+    int nfix, nres, natom, maxk;
+    int cap, ovf_base, nst, nmoves;
+    uint32_t seed;
+    int pad0;
+    const float *grid;
+    const int *moves;
+    float4 *res_pos;
+    int2 *res_rng;
+    float4 *atom;
+    int *atype;
+    const float2 *pair;
+    const Species *sp;
+    const int *fix_start;
+    const float4 *fix_rec;   // (x, y, z, gid bits)
+    int *mob_cnt, *mob_slot, *mob_loc;
+    const int *stencil;      // packed (dx+128) | (dy+128)<<8 | (dz+128)<<16
+    // per-step buffers
+    float4 *tc;              // trial centres
+    int *tslot;              // picked slot (delete/translate/rotate)
+    double *E;               // [2*maxk] E (phase 0), E_new (phase 1)
+    float4 *pose;            // [maxk*MAXF] new poses
+    uint32_t *adj;           // [maxk*ADJW] trial-centre adjacency rows
+    DevTrace tr;
+};
+
+struct State {
+    int step, end;                 // current step, exclusive end of this run
+    int kind, sp, k, nitems;       // descriptor of the current step
+    int done;                      // evaluations finished in the current step
+    int err;
+    unsigned long long claim;      // (epoch:24 | nitems:20 | claimed:20)
+    double energy;                 // running FP64 sum of accepted changes
+    int totalNum[MAXT];
+    long long att[MAXT * 4], acc[MAXT * 4];
+};
+
+struct Sched {
+    int finished;                  // replicas at the end of their run
+    int abort;                     // set on a spin-limit expiry
+    int progress;                  // completed moves (any replica)
+    int pad;
+};
+
+// byte offsets of the dynamic LDS carve-up (host computes, device applies)
+struct LdsLayout {
+    int scell, sfix, soff;         // stencil entries (int[nst+2])
+    int incg, incs, incn;          // included groups (int[INC_CAP+1])
+    int frag, frow;                // fragment pose (float4[MAXF], int[MAXF])
+    int pair;                      // float2[X*Y] or -1
+    int misc, dred;                // int[32], double[2*NWAVE]
+    int members, w, accl;          // accept: int[maxk], double[maxk], int2[maxk]
+    int J;                         // prep: int[maxk]
+    int total;
+};
+
+__host__ __device__ inline unsigned long long claim_word(int epoch, int nitems, int claimed)
+{
+    return ((unsigned long long)(uint32_t)(epoch & 0xFFFFFF) << 40) |
+           ((unsigned long long)(uint32_t)(nitems & 0xFFFFF) << 20) |
+           (unsigned long long)(uint32_t)(claimed & 0xFFFFF);
+}
+
+// cell of a position (wrapped into the box), same float code on host and GPU
+__host__ __device__ inline int cell_index(const float L[3], const float start[3],
+                                          const float inv_h[3], const int nc[3],
+                                          float x, float y, float z)
+{
+    float p[3] = {x, y, z};
+    int c[3];
+    for (int i = 0; i < 3; ++i) {
+        float w = p[i] - start[i];
+        w = w - floorf(w / L[i]) * L[i];
+        int ci = (int)(w * inv_h[i]);
+        c[i] = ci < 0 ? 0 : (ci >= nc[i] ? nc[i] - 1 : ci);
+    }
+    return (c[0] * nc[1] + c[1]) * nc[2] + c[2];
+}
+
+}  // namespace gcmc

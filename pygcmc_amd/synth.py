@@ -1,0 +1,411 @@
+"""Synthetic GCMC systems for the benchmark and parity configurations.
+
+The force-field resources of pyGCMC are not available, so every system here is
+generated from a seed: a random "protein" of LJ sites, rigid fragment
+templates at ideal geometry, CHARMM-style LJ parameters converted to the
+GROMACS units the engine uses (sigma nm, epsilon kJ/mol), and the move
+schedule / cavity grid / grouping / packing of ``pygcmc_amd.dataset``.
+
+Workloads (sizes as in the benchmark definitions):
+
+* ``water_sphere``   -- 2,000 fixed atoms in a 25 A sphere, 40 A box, TIP3P only
+                         (conc 55 M, muex -5.8, conf 1, 15 A cutoff), 1,000 steps
+* ``multi_fragment`` -- 20,000 fixed atoms, 80 A box, 8 rigid fragments at
+                         0.25 M + water, 100,000 steps
+* ``replicas``       -- 64 systems, 5k-60k atoms log-uniform, box 40*(N/2000)^(1/3)
+* ``slab``           -- 300,000-atom charged slab with a 15 A pore, 150x150x120 box
+"""
+
+import math
+import random as _random
+
+import numpy as np
+
+from . import dataset
+from .values import KCAL_TO_KJ
+
+# ---------------------------------------------------------------------------
+# fragment templates at ideal bond lengths and angles
+# ---------------------------------------------------------------------------
+
+
+def _unit(v):
+    v = np.asarray(v, dtype=np.float64)
+    return v / np.linalg.norm(v)
+
+
+def _tetra_dirs(b1, b2=None):
+    """Remaining tetrahedral bond directions at a centre with existing bonds
+    along b1 (three returned) or along b1 and b2 (two returned)."""
+    b1 = _unit(b1)
+    tet = math.acos(-1.0 / 3.0)
+    if b2 is None:
+        ref = np.array([0.0, 0.0, 1.0]) if abs(b1[2]) < 0.9 else np.array([1.0, 0.0, 0.0])
+        u = _unit(np.cross(b1, ref))
+        v = np.cross(b1, u)
+        return [math.cos(tet) * b1 + math.sin(tet) * (math.cos(2 * math.pi * k / 3) * u
+                                                     + math.sin(2 * math.pi * k / 3) * v)
+                for k in range(3)]
+    b2 = _unit(b2)
+    bis = _unit(b1 + b2)
+    n = _unit(np.cross(b1, b2))
+    half = tet / 2
+    return [-math.cos(half) * bis + math.sin(half) * n, -math.cos(half) * bis - math.sin(half) * n]
+
+
+def _methyl(c, bond_dir, length=1.09):
+    """Three H on carbon c whose single heavy-atom bond points along bond_dir."""
+    return [c + length * d for d in _tetra_dirs(bond_dir)]
+
+
+def _planar(angle_deg, length, origin, a_dir, b_dir_sign=1.0):
+    """Point at `length` from origin, at `angle_deg` from a_dir in the xy plane."""
+    a = math.atan2(a_dir[1], a_dir[0]) + b_dir_sign * math.radians(angle_deg)
+    return origin + length * np.array([math.cos(a), math.sin(a), 0.0])
+
+
+def template_water():
+    """TIP3P: OH 0.9572 A, HOH 104.52 deg."""
+    th = math.radians(104.52)
+    xyz = np.array([[0, 0, 0], [0.9572, 0, 0], [0.9572 * math.cos(th), 0.9572 * math.sin(th), 0]])
+    return xyz, ['OT', 'HT', 'HT']
+
+
+def template_benzene():
+    xyz, el = [], []
+    for k in range(6):
+        a = math.radians(60 * k)
+        xyz.append([1.39 * math.cos(a), 1.39 * math.sin(a), 0.0]); el.append('C')
+    for k in range(6):
+        a = math.radians(60 * k)
+        xyz.append([2.48 * math.cos(a), 2.48 * math.sin(a), 0.0]); el.append('H')
+    return np.array(xyz), el
+
+
+def template_propane():
+    c1 = np.zeros(3)
+    c2 = c1 + 1.53 * np.array([1.0, 0, 0])
+    th = math.radians(109.47)
+    c3 = c2 + 1.53 * np.array([-math.cos(th), math.sin(th), 0.0])
+    xyz = [c1, c2, c3]
+    xyz += _methyl(c1, c2 - c1)
+    xyz += _methyl(c3, c2 - c3)
+    xyz += [c2 + 1.09 * d for d in _tetra_dirs(c1 - c2, c3 - c2)]
+    return np.array(xyz), ['C'] * 3 + ['H'] * 8
+
+
+def template_methanol():
+    c = np.zeros(3)
+    o = np.array([1.43, 0, 0])
+    h_o = o + 0.96 * np.array([math.cos(math.radians(180 - 108.5)), math.sin(math.radians(180 - 108.5)), 0])
+    xyz = [c, o, h_o] + _methyl(c, o - c)
+    return np.array(xyz), ['C', 'O', 'H', 'H', 'H', 'H']
+
+
+def template_formamide():
+    c = np.zeros(3)
+    o = np.array([1.23, 0, 0])
+    n = _planar(120, 1.35, c, o - c)
+    h_c = _planar(-120, 1.10, c, o - c)
+    hn1 = _planar(120, 1.01, n, c - n)
+    hn2 = _planar(-120, 1.01, n, c - n)
+    return np.array([c, o, n, h_c, hn1, hn2]), ['C', 'O', 'N', 'H', 'H', 'H']
+
+
+def template_acetaldehyde():
+    c2 = np.zeros(3)
+    o = np.array([1.22, 0, 0])
+    c1 = _planar(120, 1.50, c2, o - c2)
+    h2 = _planar(-120, 1.10, c2, o - c2)
+    xyz = [c1, c2, o, h2] + _methyl(c1, c2 - c1)
+    return np.array(xyz), ['C', 'C', 'O', 'H', 'H', 'H', 'H']
+
+
+def template_imidazole():
+    r = 1.36 / (2 * math.sin(math.pi / 5))
+    ring = [np.array([r * math.cos(2 * math.pi * k / 5), r * math.sin(2 * math.pi * k / 5), 0.0])
+            for k in range(5)]
+    el = ['N', 'C', 'N', 'C', 'C']
+    hs, hel = [], []
+    for k, blen in [(0, 1.01), (1, 1.08), (3, 1.08), (4, 1.08)]:
+        hs.append(ring[k] + blen * _unit(ring[k]))
+        hel.append('H')
+    return np.array(ring + hs), el + hel
+
+
+def template_methylammonium():
+    c = np.zeros(3)
+    n = np.array([1.49, 0, 0])
+    xyz = [c, n] + _methyl(c, n - c, 1.09) + _methyl(n, c - n, 1.03)
+    return np.array(xyz), ['C', 'N'] + ['H'] * 6
+
+
+def template_acetate():
+    c2 = np.zeros(3)
+    c1 = np.array([-1.52, 0, 0])
+    o1 = _planar(117.5, 1.26, c2, c1 - c2)
+    o2 = _planar(-117.5, 1.26, c2, c1 - c2)
+    xyz = [c1, c2, o1, o2] + _methyl(c1, c2 - c1)
+    return np.array(xyz), ['C', 'C', 'O', 'O', 'H', 'H', 'H']
+
+
+# name, template builder, net charge, SILCS-style conf
+FRAGMENTS = [
+    ('BENX', template_benzene, 0, 400),
+    ('PRPX', template_propane, 0, 15),
+    ('MEOH', template_methanol, 0, 10),
+    ('FORM', template_formamide, 0, 20),
+    ('ALDX', template_acetaldehyde, 0, 15),
+    ('IMIA', template_imidazole, 0, 300),
+    ('MAMY', template_methylammonium, 1, 100),
+    ('ACEY', template_acetate, -1, 200),
+]
+
+# CHARMM-like (epsilon kcal/mol, Rmin/2 A) of the fragment element types and
+# of TIP3P (OT/HT as in CHARMM36)
+ELEMENT_LJ = {
+    'C': (0.070, 1.9924), 'H': (0.030, 1.3582), 'N': (0.200, 1.8500),
+    'O': (0.120, 1.7000), 'OT': (0.1521, 1.7682), 'HT': (0.046, 0.2245),
+}
+TIP3P_CHARGE = [-0.834, 0.417, 0.417]
+
+
+def fragment_charges(rng, n, net, sd=0.3):
+    """Partial charges N(0, sd) shifted to the integer net charge."""
+    q = rng.normal(0.0, sd, n)
+    q += (net - q.sum()) / n
+    return q
+
+
+# ---------------------------------------------------------------------------
+# fixed atoms
+# ---------------------------------------------------------------------------
+
+
+def _morton_key(ix, iy, iz):
+    key = np.zeros_like(ix, dtype=np.int64)
+    for b in range(21):
+        key |= ((ix >> b) & 1) << (3 * b + 2)
+        key |= ((iy >> b) & 1) << (3 * b + 1)
+        key |= ((iz >> b) & 1) << (3 * b)
+    return key
+
+
+def random_sphere_atoms(rng, n, radius, box, min_dist=1.2, centre=None):
+    """n points uniform in a sphere with pairwise min-image spacing >= min_dist
+    (batched random sequential addition), rounded to PDB precision."""
+    from scipy.spatial import cKDTree
+    box = np.asarray(box, dtype=np.float64)
+    centre = box / 2 if centre is None else np.asarray(centre, dtype=np.float64)
+    acc = np.zeros((0, 3))
+    while len(acc) < n:
+        m = int((n - len(acc)) * 1.3) + 64
+        u = rng.normal(size=(m, 3))
+        u /= np.linalg.norm(u, axis=1)[:, None]
+        r = radius * rng.random(m) ** (1 / 3)
+        cand = np.round(centre + u * r[:, None], 3)
+        keep = np.ones(m, dtype=bool)
+        wc = np.mod(cand, box)
+        if len(acc):
+            tree = cKDTree(np.mod(acc, box), boxsize=box)
+            d, _ = tree.query(wc, k=1)
+            keep &= d >= min_dist
+        t2 = cKDTree(wc, boxsize=box)
+        for i, j in sorted(t2.query_pairs(min_dist)):
+            if keep[i] and keep[j]:
+                keep[j] = False
+        acc = np.concatenate([acc, cand[keep]])
+    return acc[:n]
+
+
+def random_slab_atoms(rng, box, thickness, pore_radius, spacing=None, n_target=300000,
+                      min_dist=1.2):
+    """Jittered simple-cubic lattice filling a slab |z - Lz/2| <= thickness/2
+    minus a cylindrical pore along z; lattice spacing chosen to give n_target
+    atoms and jitter bounded so pairs stay >= min_dist apart."""
+    box = np.asarray(box, dtype=np.float64)
+    vol = box[0] * box[1] * thickness - math.pi * pore_radius ** 2 * thickness
+    a = spacing or (vol / n_target) ** (1 / 3)
+    jit = max(0.0, (a - min_dist) / 2 - 1e-3)
+    xs = np.arange(a / 2, box[0], a)
+    ys = np.arange(a / 2, box[1], a)
+    zs = np.arange(box[2] / 2 - thickness / 2 + a / 2, box[2] / 2 + thickness / 2, a)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing='ij')
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    rr = np.hypot(pts[:, 0] - box[0] / 2, pts[:, 1] - box[1] / 2)
+    pts = pts[rr > pore_radius]
+    pts = pts + rng.uniform(-jit, jit, pts.shape)
+    if len(pts) > n_target:
+        pts = pts[np.sort(rng.choice(len(pts), n_target, replace=False))]
+    return np.round(pts, 3)
+
+
+def morton_order(xyz, cell=6.0):
+    q = np.floor((xyz - xyz.min(axis=0)) / cell).astype(np.int64)
+    return np.argsort(_morton_key(q[:, 0], q[:, 1], q[:, 2]), kind='stable')
+
+
+# ---------------------------------------------------------------------------
+# system assembly (the in-memory counterpart of GCMC.get_simulation)
+# ---------------------------------------------------------------------------
+
+
+def build_system(*, seed, fixed_xyz, fixed_charge, fixed_type_names, protein_lj,
+                 species_specs, cryst, mcsteps, cutoff=15.0, grid_dx=1.0, fix_cutoff=6.0,
+                 cavity_bias=True, grid_order='linear', pme=0):
+    """Assemble a packed system the way ``get_simulation`` does
+    (``gcmc/python/simulation.py:16-108``): move schedule, cavity grid,
+    type tables and LJ pair table, fixed-atom grouping, packing, and the
+    device seed drawn last from the same stdlib stream.
+
+    Returns a dict with the seven runGCMC arrays and the metadata.
+    """
+    rng = _random.Random(seed)
+    names = [s['name'] for s in species_specs]
+    mctime = [s['mctime'] for s in species_specs]
+    moves, counts = dataset.get_move(names, mctime, mcsteps, rng=rng)
+
+    all_xyz = [np.asarray(fixed_xyz, dtype=np.float64)]
+    for s in species_specs:
+        for mol in s.get('molecules', []):
+            all_xyz.append(np.asarray(mol, dtype=np.float64))
+    all_xyz = np.concatenate(all_xyz) if len(all_xyz) > 1 else all_xyz[0]
+    start, grid_n, grid, cav = dataset.get_grid(all_xyz, cryst, grid_dx, cavity_bias, grid_order)
+
+    # type lists: fragment types first (gcmc/python/simulation.py:37-47)
+    types1, types2 = [], []
+    for s in species_specs:
+        for t in s['types']:
+            if t not in types2:
+                types1.append(t)
+                types2.append(t)
+    for t in fixed_type_names:
+        if t not in types2:
+            types2.append(t)
+    nb = {}
+    for s in species_specs:
+        for t, lj in zip(s['types'], s['lj']):
+            nb[t] = lj
+    nb.update(protein_lj)
+    ff = dataset.ff_pair_table(types1, types2, nb)
+    tindex = {t: i for i, t in enumerate(types2)}
+
+    groups = dataset.set_fixcut(fixed_xyz, fix_cutoff)
+    species = []
+    for s in species_specs:
+        species.append(dict(name=s['name'], conc=s['conc'], muex=s['muex'], conf=s['conf'],
+                            mctime=s['mctime'], template_xyz=s['template_xyz'],
+                            charge=s['charge'], type=[tindex[t] for t in s['types']],
+                            molecules=s.get('molecules', [])))
+    dev_seed = rng.randint(0, (2 ** 32) - 1)
+    arrays = dataset.pack_system(
+        cryst=cryst, cutoff=cutoff, grid_dx=grid_dx, startxyz=start, grid=grid,
+        cavity_factor=cav, fixed_xyz=fixed_xyz, fixed_charge=fixed_charge,
+        fixed_type=[tindex[t] for t in fixed_type_names], fixed_group=groups,
+        species=species, ff=ff, ffX=len(types1), ffY=len(types2), moves=moves,
+        move_counts=counts, mcsteps=mcsteps, seed=dev_seed, pme=pme)
+    info, frag, res, atoms, grid, ff, moves = arrays
+    return dict(info=info, frag=frag, res=res, atoms=atoms, grid=grid, ff=ff, moves=moves,
+                move_counts=counts, names=names, types1=types1, types2=types2, nb=nb,
+                grid_n=grid_n, startxyz=start, seed=dev_seed)
+
+
+def _protein(rng, n, radius, box, n_types=64, q_sd=0.4):
+    xyz = random_sphere_atoms(rng, n, radius, box)
+    xyz = xyz[morton_order(xyz)]
+    q = rng.normal(0.0, q_sd, n)
+    q -= q.mean()
+    eps = rng.uniform(0.05, 0.2, n_types)
+    rmin = rng.uniform(1.0, 2.2, n_types)
+    lj = {f'P{i:02d}': dataset.charmm_to_gromacs(eps[i], rmin[i]) for i in range(n_types)}
+    tnames = [f'P{i:02d}' for i in rng.integers(0, n_types, n)]
+    return xyz, q, tnames, lj
+
+
+def water_spec(conc=55.0, muex=-5.8, conf=1, mctime=1.0, molecules=()):
+    xyz, el = template_water()
+    return dict(name='SOL', conc=conc, muex=muex, conf=conf, mctime=mctime,
+                template_xyz=xyz, charge=list(TIP3P_CHARGE), types=list(el),
+                lj=[dataset.charmm_to_gromacs(*ELEMENT_LJ[e]) for e in el],
+                molecules=list(molecules))
+
+
+def fragment_specs(rng, conc=0.25, muex=None, confs=None, mctime=1.0, names=None):
+    out = []
+    for i, (name, builder, net, conf) in enumerate(FRAGMENTS):
+        if names is not None and name not in names:
+            continue
+        xyz, el = builder()
+        q = fragment_charges(rng, len(el), net)
+        m = float(rng.uniform(-8.0, 0.0)) if muex is None else muex[i]
+        c = conf if confs is None else confs[i]
+        out.append(dict(name=name, conc=conc, muex=m, conf=c, mctime=mctime,
+                        template_xyz=xyz, charge=list(q), types=list(el),
+                        lj=[dataset.charmm_to_gromacs(*ELEMENT_LJ[e]) for e in el],
+                        molecules=[]))
+    return out
+
+
+def water_sphere(seed=2024, n_fixed=2000, radius=25.0, box=40.0, mcsteps=1000, conf=1,
+                 cutoff=15.0, muex=-5.8):
+    """2,000 fixed atoms in a 25 A sphere, 40 A box, TIP3P water only."""
+    rng = np.random.default_rng(seed)
+    cryst = [box, box, box]
+    xyz, q, tnames, lj = _protein(rng, n_fixed, radius, cryst)
+    sp = [water_spec(muex=muex, conf=conf)]
+    return build_system(seed=seed, fixed_xyz=xyz, fixed_charge=q, fixed_type_names=tnames,
+                        protein_lj=lj, species_specs=sp, cryst=cryst, mcsteps=mcsteps,
+                        cutoff=cutoff)
+
+
+def multi_fragment(seed=2025, n_fixed=20000, box=80.0, mcsteps=100000, radius=None,
+                   cutoff=15.0, conf_scale=1.0, fragments=None):
+    """20,000 fixed atoms in an 80 A box, 8 rigid fragments at 0.25 M plus water."""
+    rng = np.random.default_rng(seed)
+    radius = radius or 25.0 * (n_fixed / 2000.0) ** (1 / 3)
+    cryst = [box, box, box]
+    xyz, q, tnames, lj = _protein(rng, n_fixed, radius, cryst)
+    frags = fragment_specs(rng, names=fragments)
+    for f in frags:
+        f['conf'] = max(1, int(round(f['conf'] * conf_scale)))
+    sp = frags + [water_spec(conf=2, mctime=10.0)]
+    return build_system(seed=seed, fixed_xyz=xyz, fixed_charge=q, fixed_type_names=tnames,
+                        protein_lj=lj, species_specs=sp, cryst=cryst, mcsteps=mcsteps,
+                        cutoff=cutoff)
+
+
+def replica_sizes(n=64, lo=5000, hi=60000, seed=3000):
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), n)).astype(np.int64)
+
+
+def replica_system(r, n_fixed, mcsteps=10000, conf_scale=1.0):
+    """Replica r of the uneven batch: own seed 3000+r, own muex, box
+    40*(N/2000)^(1/3) A, fixed-atom density of the 2,000-atom sphere system."""
+    scale = (n_fixed / 2000.0) ** (1 / 3)
+    return multi_fragment(seed=3000 + r, n_fixed=int(n_fixed), box=40.0 * scale,
+                          radius=25.0 * scale, mcsteps=mcsteps, conf_scale=conf_scale)
+
+
+def slab(seed=5000, n_fixed=300000, box=(150.0, 150.0, 120.0), thickness=40.0,
+         pore_radius=15.0, mcsteps=50000, cutoff=12.0, pme=1):
+    """300,000-atom slab with a 15 A pore: charges +-0.5 e alternating in random
+    8-atom groups, water + 0.15 M methylammonium/acetate."""
+    rng = np.random.default_rng(seed)
+    cryst = list(box)
+    xyz = random_slab_atoms(rng, cryst, thickness, pore_radius, n_target=n_fixed)
+    xyz = xyz[morton_order(xyz)]
+    n = len(xyz)
+    ng = (n + 7) // 8
+    signs = rng.choice([-0.5, 0.5], ng)
+    q = np.repeat(signs, 8)[:n]
+    q -= q.mean()
+    eps = rng.uniform(0.05, 0.2, 64)
+    rmin = rng.uniform(1.0, 2.2, 64)
+    lj = {f'P{i:02d}': dataset.charmm_to_gromacs(eps[i], rmin[i]) for i in range(64)}
+    tnames = [f'P{i:02d}' for i in rng.integers(0, 64, n)]
+    frags = fragment_specs(rng, conc=0.15, names=['MAMY', 'ACEY'])
+    sp = frags + [water_spec(conf=2, mctime=10.0)]
+    return build_system(seed=seed, fixed_xyz=xyz, fixed_charge=q, fixed_type_names=tnames,
+                        protein_lj=lj, species_specs=sp, cryst=cryst, mcsteps=mcsteps,
+                        cutoff=cutoff, pme=pme)

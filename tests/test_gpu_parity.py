@@ -1,0 +1,165 @@
+"""GPU parity of the HIP engine against the FP64 oracle (same variates).
+
+Bars (per-move energies in kJ/mol; 1e-4 kcal/mol = 4.184e-4 kJ/mol):
+* trial centres, poses, included-group sets, slots: bit-exact;
+* trial energies: |E_gpu - E_fp64| <= 4.184e-4 + 4e-7 |E| (the relative term
+  only matters for strongly repulsive overlaps, whose weights are ~0);
+* accept/reject decisions and picks: identical on >= 99.99% of groups;
+* final state after a lock-step replay: bit-identical.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pygcmc_amd.engine import Engine
+
+from . import systems
+
+pytestmark = pytest.mark.gpu
+
+KCAL = 4.184e-4
+
+
+def e_tol(e):
+    return KCAL + 4e-7 * np.abs(e)
+
+
+def test_eval_poses_matches_fp64():
+    s = systems.populated_water()
+    eng = Engine()
+    eng.add_system(s)
+    eng.upload()
+    c, p = systems.random_poses(s, 0, 512, seed=7)
+    E, N, H = eng.eval_poses(0, 0, c, p)
+    for i in range(len(c)):
+        e, n, h = O.energy_pose(s, 0, c[i], p[i])
+        assert n == N[i] and h == H[i], (i, n, N[i])
+        assert abs(e - E[i]) <= e_tol(e), (i, e, E[i])
+    eng.close()
+
+
+def _lockstep(sysd, nsteps, mode='plain', max_k=None):
+    max_k = max_k or int(sysd['frag']['confBias'].max())
+    gpu_sys = systems.copy(sysd)
+    eng = Engine()
+    eng.add_system(gpu_sys)
+    eng.upload()
+    eng.set_trace(0, nsteps, max_k)
+    eng.run(nsteps, mode)
+    gt = eng.get_trace(0)
+    eng.download(0)
+    st = eng.stats(0)
+    eng.close()
+    ref = systems.copy(sysd)
+    ot = O.new_trace(nsteps, max_k)
+    O.run(ref, first=0, nsteps=nsteps, trace=ot, follow=gt)
+    return gt, ot, gpu_sys, ref, st
+
+
+def _compare(gt, ot, gpu_sys, ref):
+    assert np.array_equal(gt['k'], ot['k'])
+    kmask = np.arange(gt['centre'].shape[1])[None, :] < gt['k'][:, None]
+    assert np.array_equal(gt['centre'][kmask], ot['centre'][kmask])
+    assert np.array_equal(gt['posehash'][kmask], ot['posehash'][kmask])
+    assert np.array_equal(gt['ninc'][kmask], ot['ninc'][kmask])
+    assert np.array_equal(gt['inchash'][kmask], ot['inchash'][kmask])
+    eg, eo = gt['e'][kmask], ot['e'][kmask]
+    err = np.abs(eg - eo)
+    assert (err <= e_tol(eo)).all(), (err.max(), eo[err.argmax()])
+    gmask = np.arange(gt['gpick'].shape[1])[None, :] < gt['ngroups'][:, None]
+    ng = int(gmask.sum())
+    mism = int((gt['gpick'][gmask] != ot['gpick'][gmask]).sum() +
+               (gt['gacc'][gmask] != ot['gacc'][gmask]).sum())
+    assert mism <= max(0, int(1e-4 * ng)), (mism, ng)
+    assert np.array_equal(gt['naccept'], ot['naccept']) or mism > 0
+    # lock-step replay leaves bit-identical states
+    assert np.array_equal(gpu_sys['frag']['totalNum'], ref['frag']['totalNum'])
+    assert np.array_equal(gpu_sys['res']['position'].view(np.uint32), ref['res']['position'].view(np.uint32))
+    assert np.array_equal(gpu_sys['atoms'].view(np.uint8), ref['atoms'].view(np.uint8))
+    return float(err.max()), ng, mism
+
+
+def test_lockstep_water_plain():
+    s = systems.water(nsteps=600, conf=3)
+    gt, ot, g, r, st = _lockstep(s, 600, 'plain')
+    err, ng, mism = _compare(gt, ot, g, r)
+    print(f'water plain: max |dE| {err:.3e} kJ/mol over {ng} groups, {mism} mismatches, '
+          f'N={g["frag"]["totalNum"]}')
+    assert g['frag']['totalNum'][0] > 0
+
+
+def test_lockstep_water_batched():
+    s = systems.water(nsteps=600, conf=3)
+    gt, ot, g, r, st = _lockstep(s, 600, 'batched')
+    err, ng, mism = _compare(gt, ot, g, r)
+    print(f'water batched: max |dE| {err:.3e}, {ng} groups, {mism} mismatches')
+
+
+def test_lockstep_multi_fragment():
+    s = systems.multi(nsteps=400)
+    gt, ot, g, r, st = _lockstep(s, 400, 'batched')
+    err, ng, mism = _compare(gt, ot, g, r)
+    print(f'multi batched: max |dE| {err:.3e}, {ng} groups, {mism} mismatches, N={g["frag"]["totalNum"]}')
+
+
+def test_plain_batched_graph_identical():
+    s = systems.multi(nsteps=300, seed=11)
+    outs = []
+    for mode in ('plain', 'batched', 'graph'):
+        g = systems.copy(s)
+        eng = Engine()
+        eng.add_system(g)
+        eng.upload()
+        eng.set_trace(0, 300, int(s['frag']['confBias'].max()))
+        for _ in range(3):
+            eng.run(100, mode)
+        tr = eng.get_trace(0)
+        eng.download(0)
+        st = eng.stats(0)
+        eng.close()
+        outs.append((tr, g, st))
+    for tr, g, st in outs[1:]:
+        assert np.array_equal(tr['e'].view(np.uint64), outs[0][0]['e'].view(np.uint64))
+        assert np.array_equal(tr['gacc'], outs[0][0]['gacc'])
+        assert np.array_equal(g['atoms'].view(np.uint8), outs[0][1]['atoms'].view(np.uint8))
+        assert st['energy'] == outs[0][2]['energy']
+
+
+def test_replicas_match_single_runs():
+    """Uneven replicas in one persistent kernel give the same chains as
+    each system run alone."""
+    sizes = [(1000, 30.0, 21), (2500, 45.0, 22), (1600, 38.0, 23)]
+    base = [systems.multi(nsteps=200, n_fixed=n, box=b, seed=sd) for n, b, sd in sizes]
+    eng = Engine()
+    batch = [systems.copy(s) for s in base]
+    for b in batch:
+        eng.add_system(b)
+    eng.upload()
+    eng.run(200, 'batched')
+    for r in range(len(batch)):
+        eng.download(r)
+    eng.close()
+    for s, b in zip(base, batch):
+        one = systems.copy(s)
+        e1 = Engine()
+        e1.add_system(one)
+        e1.upload()
+        e1.run(200, 'plain')
+        e1.download(0)
+        e1.close()
+        assert np.array_equal(one['atoms'].view(np.uint8), b['atoms'].view(np.uint8))
+        assert np.array_equal(one['frag']['totalNum'], b['frag']['totalNum'])
+
+
+def test_rungcmc_dropin_matches_oracle():
+    """pygcmc_amd.gpu.runGCMC: same arrays, same in-place contract as gcmc.gpu.runGCMC."""
+    from pygcmc_amd import gpu
+    s = systems.water(nsteps=400, conf=2)
+    g = systems.copy(s)
+    gpu.runGCMC(g['info'], g['frag'], g['res'], g['atoms'], g['grid'], g['ff'], g['moves'])
+    ref = systems.copy(s)
+    O.run(ref)
+    assert np.array_equal(g['frag']['totalNum'], ref['frag']['totalNum'])
+    assert np.array_equal(g['atoms'].view(np.uint8), ref['atoms'].view(np.uint8))
+    assert np.array_equal(g['res'].view(np.uint8), ref['res'].view(np.uint8))
