@@ -1,0 +1,225 @@
+"""Benchmark: GCMC MC steps/s and trial energy evaluations/s on one GPU.
+
+Workload (default): 64 independent GCMC systems whose fixed-atom counts are
+log-uniform in 5,000-60,000 (box 40*(N/2000)^(1/3) A, fixed-atom density of the
+2,000-atom sphere system), each with its own seed and muex and the 8 rigid
+fragment types + TIP3P water (confBias 400/15/10/20/15/300/100/200/2), 10,000
+moves each, FP32 pair energies with compensated sums and FP64 totals and
+acceptance, all systems in one persistent kernel.  Synthetic data (seeded).
+
+Prints one JSON line.  `value` is the batched form's aggregate MC steps/s over
+all systems with inputs resident in HBM; the plain form (one move of one
+system per launch pair) is timed on the same moves beside it.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_PAIR = 32         # algorithmic count per fragment-environment pair
+BYTES_PER_ITEM = 20        # per included env atom / included group / fragment atom
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_replicas(n, steps):
+    from pygcmc_amd import synth
+    sizes = synth.replica_sizes(n)
+    out = []
+    for r, na in enumerate(sizes):
+        out.append(synth.replica_system(r, int(na), mcsteps=steps))
+    return out, sizes
+
+
+def copies(systems):
+    return [{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in s.items()} for s in systems]
+
+
+def make_engine(systems, tuning):
+    from pygcmc_amd.engine import Engine
+    eng = Engine(**tuning)
+    for s in systems:
+        eng.add_system(s)
+    eng.upload()
+    return eng
+
+
+def counters_sum(eng, R):
+    tot = np.zeros(5, dtype=np.float64)
+    for r in range(R):
+        c = eng.counters(r)
+        tot += [c['evals'], c['pairs'], c['atoms'], c['groups'], c['frag_atoms']]
+    return tot
+
+
+def cpu_baseline(states, first, budget_s):
+    """The FP64 oracle (single thread, reference algorithm: all groups scanned
+    per trial) continuing the same chains from step `first`."""
+    from oracle import oracle as O
+    done = 0
+    t0 = time.perf_counter()
+    for s in states:
+        step = first
+        while time.perf_counter() - t0 < budget_s and step < len(s['moves']):
+            O.run(s, first=step, nsteps=1, nthreads=1)
+            step += 1
+            done += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return done / dt, done, dt
+
+
+def parity_sample(system, nsteps):
+    """Max |dE| of the GPU against the FP64 oracle on one system (lock-step)."""
+    from oracle import oracle as O
+    from pygcmc_amd.engine import Engine
+    g = copies([system])[0]
+    k = int(g['frag']['confBias'].max())
+    eng = Engine()
+    eng.add_system(g)
+    eng.upload()
+    eng.set_trace(0, nsteps, k)
+    eng.run(nsteps, 'batched')
+    tr = eng.get_trace(0)
+    eng.close()
+    ref = copies([system])[0]
+    ot = O.new_trace(nsteps, k)
+    O.run(ref, first=0, nsteps=nsteps, trace=ot, follow=tr)
+    m = np.arange(k)[None, :] < tr['k'][:, None]
+    e_ref = ot['e'][m]
+    err = np.abs(tr['e'][m] - e_ref)
+    mod = np.abs(e_ref) <= 1000.0
+    gm = np.arange(k)[None, :] < tr['ngroups'][:, None]
+    agree = float((tr['gacc'][gm] == ot['gacc'][gm]).mean()) if gm.any() else 1.0
+    return float(err[mod].max()) if mod.any() else 0.0, agree, int(gm.sum())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=9800)
+    ap.add_argument('--warmup', type=int, default=200)
+    ap.add_argument('--replicas', type=int, default=64)
+    ap.add_argument('--plain-steps', type=int, default=100)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0)
+    ap.add_argument('--parity-steps', type=int, default=300)
+    ap.add_argument('--mode', default='graph', choices=['batched', 'graph'])
+    ap.add_argument('--cell-size', type=float, default=0.0)
+    ap.add_argument('--wgs', type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported in this build '
+              '(independent systems would shard one group per GPU, see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+
+    total = args.warmup + args.steps
+    t0 = time.time()
+    systems, sizes = build_replicas(args.replicas, total)
+    R = len(systems)
+    log(f'built {R} systems ({int(sizes.sum())} fixed atoms) in {time.time() - t0:.1f}s')
+    tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs)
+
+    # batched form over the whole schedule
+    eng = make_engine(copies(systems), tuning)
+    if args.warmup:
+        eng.run(args.warmup, 'batched')
+    states_w = [eng.download(r, copies([systems[r]])[0]) for r in range(R)]
+    c0 = counters_sum(eng, R)
+    ms_total = eng.run(args.steps, 'batched')
+    ms_kernel = eng.last_kernel_ms()
+    c1 = counters_sum(eng, R)
+    eng.close()
+    steps_done = R * args.steps
+    value = steps_done / (ms_total / 1e3)
+    evals, pairs, atoms, groups, fatoms = c1 - c0
+    log(f'batched: {steps_done} steps in {ms_total:.1f} ms (kernel {ms_kernel:.1f} ms)')
+
+    # graph replay of the same chains (timing only, from the warm state)
+    graph_rate = None
+    if args.mode == 'graph':
+        eg = make_engine(copies(states_w), tuning)
+        gsteps = min(args.steps, 2000)
+        ms_g = 0.0
+        done = 0
+        while done < gsteps:
+            n = min(500, gsteps - done)
+            ms_g += eg.run(n, 'graph')
+            done += n
+        eg.close()
+        graph_rate = R * gsteps / (ms_g / 1e3)
+        log(f'graph: {R * gsteps} steps in {ms_g:.1f} ms')
+
+    # plain vs batched on the same first moves
+    ps = args.plain_steps
+    ep = make_engine(copies(systems), tuning)
+    ms_plain = ep.run(ps, 'plain')
+    ep.close()
+    eb = make_engine(copies(systems), tuning)
+    ms_b = eb.run(ps, 'batched')
+    eb.close()
+    plain_rate = R * ps / (ms_plain / 1e3)
+    batched_same = R * ps / (ms_b / 1e3)
+    log(f'plain: {R * ps} steps in {ms_plain:.1f} ms; batched same moves {ms_b:.1f} ms')
+
+    # accuracy on one system against the FP64 oracle
+    max_err, agree, ngroups = parity_sample(systems[0], args.parity_steps)
+    log(f'parity: max |dE| {max_err:.2e} kJ/mol, decisions agree {agree:.6f} over {ngroups} groups')
+
+    # CPU baseline: the oracle continuing the warm chains
+    cpu_rate, cpu_done, cpu_dt = cpu_baseline(copies(states_w[:4]), args.warmup, args.cpu_seconds)
+    log(f'cpu: {cpu_done} steps in {cpu_dt:.1f}s')
+
+    tflops = FLOP_PER_PAIR * pairs / (ms_kernel / 1e3) / 1e12
+    gbs = BYTES_PER_ITEM * (atoms + groups + fatoms) / (ms_kernel / 1e3) / 1e9
+    out = {
+        'metric': 'MC steps/sec + trial dE evals/sec, batched vs plain, at |dE| err <= 1e-4 kcal/mol vs FP64',
+        'value': round(value, 1),
+        'unit': 'MC steps/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_total / args.steps,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'fp32 pairs + fp64 totals',
+        'data': 'synthetic (seeded)',
+        'config': {'workload': f'{R} uneven GCMC replicas (5k-60k fixed atoms, 8 fragments + water, '
+                               f'{total} moves each)', 'replicas': R,
+                   'fixed_atoms_total': int(sizes.sum()), 'moves_per_replica': total,
+                   'parallelism': 'replicas in one persistent kernel'},
+        'trial_evals_per_s': round(evals / (ms_total / 1e3), 1),
+        'pairs_per_s': pairs / (ms_total / 1e3),
+        'graph_steps_per_s': None if graph_rate is None else round(graph_rate, 1),
+        'plain_steps_per_s': round(plain_rate, 1),
+        'batched_steps_per_s_same_moves': round(batched_same, 1),
+        'batched_over_plain': round(batched_same / plain_rate, 2),
+        'max_abs_dE_kJ': max_err,
+        'max_abs_dE_kcal': max_err / 4.184,
+        'decision_agreement': agree,
+        'roofline': {'bound': 'mfma', 'pipe': 'fp32-valu', 'kernel': 'k_batched',
+                     'achieved': round(tflops, 3), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': round(tflops / FP32_PEAK_TFLOPS, 4), 'traffic': None,
+                     'algorithmic_GBps': round(gbs, 1), 'kernel_ms': round(ms_kernel, 2),
+                     'pairs': int(pairs), 'evals': int(evals)},
+        'cpu_baseline': {'value': round(cpu_rate, 2), 'unit': 'MC steps/s', 'cores': 1, 'kind': 'port',
+                         'sample': f'FP64 oracle, {cpu_done} moves of systems 0-3 continuing from '
+                                   f'move {args.warmup} ({cpu_dt:.1f} s)'},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -114,8 +114,15 @@ int gcmc_engine_download(gcmc_engine *e, int replica, AtomArray *frag, residue *
  * (attempts/accepts: fragTypeNum*4 entries, index species*4 + kind). */
 int gcmc_engine_stats(gcmc_engine *e, int replica, double *energy, long long *attempts,
                       long long *accepts);
-/* Milliseconds of the last gcmc_engine_run measured with HIP events. */
+/* Milliseconds of the last gcmc_engine_run measured with HIP events on the
+ * engine's stream: the whole run, and the persistent kernel alone (batched
+ * mode; 0 otherwise). */
 int gcmc_engine_last_run_ms(gcmc_engine *e, double *ms);
+int gcmc_engine_last_kernel_ms(gcmc_engine *e, double *ms);
+/* Work done by a replica since upload: [0] trial evaluations, [1] fragment-
+ * environment atom pairs, [2] included environment atoms, [3] included
+ * groups, [4] fragment atoms, each summed over evaluations. */
+int gcmc_engine_counters(gcmc_engine *e, int replica, unsigned long long *out5);
 
 /* ---- step trace (parity tests) --------------------------------------------
  * Per step s (relative to the step when tracing was enabled) and trial t:

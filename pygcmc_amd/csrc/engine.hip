@@ -129,8 +129,10 @@ struct gcmc_engine {
     int nwg = 0;
     int maxk = 1;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
     double last_ms = 0.0;
+    double kernel_ms = 0.0;   // duration of the last persistent kernel (batched mode)
+    bool timed_kernel = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_nsteps = -1;
@@ -442,8 +444,10 @@ void launch_batched(gcmc_engine *E, int nsteps)
                        nsteps, E->d_sched);
     hipLaunchKernelGGL(k_start<S>, dim3(R), dim3(WG), lds, E->stream, E->d_rep, E->d_state, E->d_sched,
                        E->lay, 0, 1);
+    if (E->timed_kernel) (void)hipEventRecord(E->evk0, E->stream);
     hipLaunchKernelGGL(k_batched<S>, dim3(E->nwg), dim3(WG), lds, E->stream, E->d_rep, E->d_state, R,
                        E->d_sched, E->lay, E->spin_limit);
+    if (E->timed_kernel) (void)hipEventRecord(E->evk1, E->stream);
 }
 
 template <bool S>
@@ -501,6 +505,8 @@ int gcmc_engine_create(gcmc_engine **out, const gcmc_tuning *tuning)
     hipError_t e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&E->ev0);
     if (e == hipSuccess) e = hipEventCreate(&E->ev1);
+    if (e == hipSuccess) e = hipEventCreate(&E->evk0);
+    if (e == hipSuccess) e = hipEventCreate(&E->evk1);
     if (e != hipSuccess) {
         delete E;
         return fail(GCMC_E_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
@@ -525,6 +531,8 @@ int gcmc_engine_destroy(gcmc_engine *E)
     if (E->d_sched) hipFree(E->d_sched);
     if (E->ev0) hipEventDestroy(E->ev0);
     if (E->ev1) hipEventDestroy(E->ev1);
+    if (E->evk0) hipEventDestroy(E->evk0);
+    if (E->evk1) hipEventDestroy(E->evk1);
     if (E->stream) hipStreamDestroy(E->stream);
     delete E;
     return 0;
@@ -605,8 +613,10 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
         if (E->staged) launch_plain<true>(E, nsteps);
         else launch_plain<false>(E, nsteps);
     } else if (mode == GCMC_MODE_BATCHED) {
+        E->timed_kernel = true;
         if (E->staged) launch_batched<true>(E, nsteps);
         else launch_batched<false>(E, nsteps);
+        E->timed_kernel = false;
     } else if (mode == GCMC_MODE_GRAPH) {
         if (!E->gexec || E->graph_nsteps != nsteps) {
             if (E->gexec) {
@@ -631,6 +641,11 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
     float ms = 0.f;
     HIPX(hipEventElapsedTime(&ms, E->ev0, E->ev1));
     E->last_ms = ms;
+    E->kernel_ms = 0.0;
+    if (mode == GCMC_MODE_BATCHED) {
+        HIPX(hipEventElapsedTime(&ms, E->evk0, E->evk1));
+        E->kernel_ms = ms;
+    }
     std::vector<State> st;
     int rc = read_states(E, st);
     if (rc) return rc;
@@ -664,6 +679,28 @@ int gcmc_engine_last_run_ms(gcmc_engine *E, double *ms)
 {
     if (!E || !ms) return fail(GCMC_E_ARG, "null argument");
     *ms = E->last_ms;
+    return 0;
+}
+
+int gcmc_engine_last_kernel_ms(gcmc_engine *E, double *ms)
+{
+    if (!E || !ms) return fail(GCMC_E_ARG, "null argument");
+    *ms = E->kernel_ms;
+    return 0;
+}
+
+int gcmc_engine_counters(gcmc_engine *E, int r, unsigned long long *out5)
+{
+    if (!E || !E->uploaded || !out5) return fail(GCMC_E_STATE, "engine not uploaded or null output");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HIPX(hipStreamSynchronize(E->stream));
+    State S;
+    HIPX(hipMemcpy(&S, E->d_state + r, sizeof(S), hipMemcpyDeviceToHost));
+    out5[0] = S.n_eval;
+    out5[1] = S.n_pair;
+    out5[2] = S.n_atom;
+    out5[3] = S.n_group;
+    out5[4] = S.n_fatom;
     return 0;
 }
 

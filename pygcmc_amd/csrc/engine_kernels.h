@@ -111,11 +111,21 @@ __device__ inline int bsearch_le(const int *a, int n, int m)
     return lo;
 }
 
+// a - b = s + e exactly (Knuth TwoSum of a and -b); returns s
+__device__ inline float two_diff(float a, float b, float &e)
+{
+    const float s = a - b;
+    const float bv = s - a;
+    const float av = s - bv;
+    e = (a - av) - (b + bv);
+    return s;
+}
+
 // ---- pair energies of the buffered included groups ------------------------
 // lane <-> environment atom (float4 load, coalesced within a group), loop
 // over the fragment atoms held in LDS; FP32 terms, TwoSum into (acc, cmp).
 template <bool STAGED>
-__device__ inline void pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
+__device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
 {
     const int A = wg_scan(L.incn, n, L.misc);
     const float2 *PT = STAGED ? L.pair : C.pair;
@@ -129,11 +139,18 @@ __device__ inline void pair_pass(const Rep &C, const Lds &L, int n, int nf, floa
         for (int f = 0; f < nf; ++f) {
             const float4 fr = L.frag[f];
             const float2 pp = PT[L.frow[f] + ty];
-            float dx = at.x - fr.x, dy = at.y - fr.y, dz = at.z - fr.z;
+            // displacement as an exact sum s + e (TwoSum); the image shift
+            // s - n L is exact for |n| <= 1 (Sterbenz)
+            float ex, ey, ez;
+            float dx = two_diff(at.x, fr.x, ex);
+            float dy = two_diff(at.y, fr.y, ey);
+            float dz = two_diff(at.z, fr.z, ez);
             dx = fmaf(-rintf(dx * ix), Lx, dx);
             dy = fmaf(-rintf(dy * iy), Ly, dy);
             dz = fmaf(-rintf(dz * iz), Lz, dz);
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float r2h = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float r2c = fmaf(dz, ez, fmaf(dy, ey, dx * ex));
+            const float r2 = fmaf(2.0f, r2c, r2h);
             float ir = __builtin_amdgcn_rsqf(r2);
             ir = ir * fmaf(-0.5f * r2 * ir, ir, 1.5f);     // one Newton step
             const float ir2 = ir * ir;
@@ -150,6 +167,7 @@ __device__ inline void pair_pass(const Rep &C, const Lds &L, int n, int nf, floa
         }
     }
     __syncthreads();
+    return A;
 }
 
 // ---- calcEnergy over the cell list -----------------------------------------
@@ -158,7 +176,7 @@ __device__ inline void pair_pass(const Rep &C, const Lds &L, int n, int nf, floa
 // distanceP test passes (gcmc_energy.h:152-191).  Thread 0 receives results.
 template <bool STAGED>
 __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, int nf, int exclude,
-                                   const Lds &L, double *outE, int *outN, uint32_t *outH)
+                                   const Lds &L, double *outE, int *outN, uint32_t *outH, int *outA)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cc = cell_index(C.L, C.start, C.inv_h, C.nc, cx, cy, cz);
@@ -191,7 +209,7 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
     __syncthreads();
     const int M = wg_scan(L.soff, ne, L.misc);
     float acc = 0.0f, cmp = 0.0f;
-    int nin = 0;
+    int nin = 0, natom = 0;
     uint32_t hs = 0;
     if (tid == 0) L.misc[16] = 0;
     __syncthreads();
@@ -242,7 +260,7 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
         __syncthreads();
         const int ninc = L.misc[16];
         if (ninc > INC_CAP - WG || base + WG >= M) {
-            if (ninc > 0) pair_pass<STAGED>(C, L, ninc, nf, acc, cmp);
+            if (ninc > 0) natom += pair_pass<STAGED>(C, L, ninc, nf, acc, cmp);
             if (tid == 0) L.misc[16] = 0;
             __syncthreads();
         }
@@ -272,6 +290,7 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
         *outE = E;
         *outN = N;
         *outH = H;
+        *outA = natom;
     }
     __syncthreads();
 }
@@ -389,11 +408,17 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
     }
 
     double E;
-    int nin;
+    int nin, nat;
     uint32_t hh;
-    eval_energy<STAGED>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh);
+    eval_energy<STAGED>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat);
     if (tid == 0) {
         C.E[phase * k + t] = E;
+        State &W = const_cast<State &>(S);
+        atomicAdd(&W.n_eval, 1ull);
+        atomicAdd(&W.n_pair, (unsigned long long)nat * (unsigned long long)nf);
+        atomicAdd(&W.n_atom, (unsigned long long)nat);
+        atomicAdd(&W.n_group, (unsigned long long)nin);
+        atomicAdd(&W.n_fatom, (unsigned long long)nf);
         int row;
         if (tracing(C, step, row) && t < C.tr.max_k) {
             const int q = row * C.tr.max_k + t;
@@ -946,9 +971,9 @@ __global__ __launch_bounds__(WG) void k_eval_poses(const Rep *reps, int r, int s
     }
     __syncthreads();
     double e;
-    int n;
+    int n, na;
     uint32_t h;
-    eval_energy<STAGED>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n, &h);
+    eval_energy<STAGED>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n, &h, &na);
     if (tid == 0) {
         E[i] = e;
         ninc[i] = n;

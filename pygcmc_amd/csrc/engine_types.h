@@ -92,6 +92,9 @@ struct State {
     double energy;                 // running FP64 sum of accepted changes
     int totalNum[MAXT];
     long long att[MAXT * 4], acc[MAXT * 4];
+    // work counters: evaluations, fragment-environment pairs, included
+    // environment atoms and groups, fragment atoms (summed over evaluations)
+    unsigned long long n_eval, n_pair, n_atom, n_group, n_fatom;
 };
 
 struct Sched {

@@ -226,25 +226,28 @@ def pack_system(*, cryst, cutoff, grid_dx, startxyz, grid, cavity_factor,
         res[:n_fixed_groups]['atomNum'] = counts
         res[:n_fixed_groups]['atomStart'] = starts
 
-    seq = n_fixed_groups - 1
+    seq = n_fixed_groups
     atom_cursor = n_fixed
     for i, sp in enumerate(species):
-        frag[i]['startRes'] = seq + 1
+        frag[i]['startRes'] = seq
         nf = int(frag[i]['num_atoms'])
-        for j in range(int(frag[i]['maxNum'])):
-            seq += 1
-            res[seq]['atomNum'] = nf
-            res[seq]['atomStart'] = atom_cursor
-            if j < frag[i]['totalNum']:
-                mol = np.asarray(sp['molecules'][j], dtype=np.float64)
-                p = np.zeros(3, dtype=np.float32)
-                for k in range(nf):
-                    p = (p.astype(np.float64) + mol[k]).astype(np.float32)
-                    atoms[atom_cursor + k]['position'] = mol[k]
-                    atoms[atom_cursor + k]['charge'] = sp['charge'][k]
-                    atoms[atom_cursor + k]['type'] = sp['type'][k]
-                res[seq]['position'] = (p.astype(np.float64) / nf).astype(np.float32)
-            atom_cursor += nf
+        m = int(frag[i]['maxNum'])
+        slots = np.arange(m)
+        res[seq:seq + m]['atomNum'] = nf
+        res[seq:seq + m]['atomStart'] = atom_cursor + slots * nf
+        n0 = int(frag[i]['totalNum'])
+        if n0:
+            mol = np.asarray(sp['molecules'], dtype=np.float64).reshape(n0, nf, 3)
+            p = np.zeros((n0, 3), dtype=np.float32)
+            for k in range(nf):  # FP64 add into FP32 storage, atom by atom
+                p = (p.astype(np.float64) + mol[:, k]).astype(np.float32)
+            idx = (atom_cursor + slots[:n0, None] * nf + np.arange(nf)[None, :]).ravel()
+            atoms['position'][idx] = mol.reshape(-1, 3)
+            atoms['charge'][idx] = np.tile(np.asarray(sp['charge'], np.float32), n0)
+            atoms['type'][idx] = np.tile(np.asarray(sp['type'], np.int32), n0)
+            res[seq:seq + n0]['position'] = (p.astype(np.float64) / nf).astype(np.float32)
+        seq += m
+        atom_cursor += m * nf
 
     info = np.zeros(1, dtype=Info_dtype)
     info[0]['ffXNum'] = ffX

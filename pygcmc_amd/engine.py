@@ -70,6 +70,8 @@ def lib():
         L.gcmc_engine_download.argtypes = [P, I, P, P, P]
         L.gcmc_engine_stats.argtypes = [P, I, P, P, P]
         L.gcmc_engine_last_run_ms.argtypes = [P, P]
+        L.gcmc_engine_last_kernel_ms.argtypes = [P, P]
+        L.gcmc_engine_counters.argtypes = [P, I, P]
         L.gcmc_engine_set_trace.argtypes = [P, I, I, I]
         L.gcmc_engine_get_trace.argtypes = [P, I, ctypes.POINTER(CTrace)]
         L.gcmc_engine_eval_poses.argtypes = [P, I, I, I, P, P, P, P, P, P]
@@ -161,6 +163,18 @@ class Engine:
         ms = ctypes.c_double()
         _check(lib().gcmc_engine_last_run_ms(self._h, ctypes.byref(ms)))
         return ms.value
+
+    def last_kernel_ms(self):
+        """Duration of the last persistent kernel (batched mode), HIP events."""
+        ms = ctypes.c_double()
+        _check(lib().gcmc_engine_last_kernel_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def counters(self, r=0):
+        """Work since upload: evals, pairs, env atoms, groups, fragment atoms."""
+        out = np.zeros(5, np.uint64)
+        _check(lib().gcmc_engine_counters(self._h, r, _p(out)))
+        return dict(zip(('evals', 'pairs', 'atoms', 'groups', 'frag_atoms'), (int(x) for x in out)))
 
     def step(self, r=0):
         return _check(lib().gcmc_engine_step(self._h, r))
