@@ -123,6 +123,10 @@ int gcmc_engine_last_kernel_ms(gcmc_engine *e, double *ms);
  * environment atom pairs, [2] included environment atoms, [3] included
  * groups, [4] fragment atoms, each summed over evaluations. */
 int gcmc_engine_counters(gcmc_engine *e, int replica, unsigned long long *out5);
+/* Device wall-clock ticks (100 MHz) a replica spent, batched form: [0] in
+ * evaluations (summed over workgroups), [1] in accept phases, [2] in prep
+ * phases, [3] from publishing a move to its last evaluation finishing. */
+int gcmc_engine_phase_ticks(gcmc_engine *e, int replica, unsigned long long *out4);
 
 /* ---- step trace (parity tests) --------------------------------------------
  * Per step s (relative to the step when tracing was enabled) and trial t:
@@ -160,6 +164,14 @@ int gcmc_engine_eval_poses(gcmc_engine *e, int replica, int species, int npose,
  * distanceP(a_i, b_i, cryst) <= cutoff (gcmc/cpp/gcmc_energy.h:115-122). */
 int gcmc_host_within(int n, const float *a, const float *b, const float *cryst, float cutoff,
                      unsigned char *out);
+
+/* Host check of the cell-list superset property for n (centre, group) pairs:
+ * out[i] = 1 when the group's cell is one of the stencil cells the energy
+ * kernel scans around the centre's cell (box cryst from startxyz, target
+ * cell edge cell_size).  Every pair that passes gcmc_host_within must give 1. */
+int gcmc_host_stencil_covers(int n, const float *centres, const float *groups, const float *cryst,
+                             const float *startxyz, float cutoff, float cell_size,
+                             unsigned char *out);
 
 #ifdef __cplusplus
 }

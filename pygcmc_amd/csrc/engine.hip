@@ -137,6 +137,8 @@ struct gcmc_engine {
     hipGraphExec_t gexec = nullptr;
     int graph_nsteps = -1;
     int spin_limit = 1 << 22;  // polls without progress anywhere (~seconds)
+    XcdMap xm{};
+    int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]
 };
 
 namespace {
@@ -162,6 +164,18 @@ int check_system(const InfoStruct *info, const AtomArray *frag, int nmoves)
             return fail(GCMC_E_ARG, "species slots outside residueInfo");
     }
     return 0;
+}
+
+constexpr float STENCIL_EPS = 0.05f;  // A of slack over FP32 rounding of the group test
+
+void cell_geometry(const float cryst[3], float cell_size, int nc[3], float inv_h[3])
+{
+    const float h = cell_size > 0 ? cell_size : 5.0f;
+    for (int i = 0; i < 3; ++i) {
+        nc[i] = std::max(1, (int)std::floor(cryst[i] / h));
+        nc[i] = std::min(nc[i], 255);
+        inv_h[i] = (float)nc[i] / cryst[i];
+    }
 }
 
 // cell stencil: offsets whose cell box lies within rc + eps of the centre cell
@@ -240,12 +254,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     R.maxk = H.maxk;
 
     // cell list geometry
-    const float hcell = E->tune.cell_size > 0 ? E->tune.cell_size : 5.0f;
-    for (int i = 0; i < 3; ++i) {
-        R.nc[i] = std::max(1, (int)std::floor(I.cryst[i] / hcell));
-        R.nc[i] = std::min(R.nc[i], 255);
-        R.inv_h[i] = (float)R.nc[i] / I.cryst[i];
-    }
+    cell_geometry(I.cryst, E->tune.cell_size, R.nc, R.inv_h);
     R.ncell = R.nc[0] * R.nc[1] * R.nc[2];
     const float h3 = (I.cryst[0] / R.nc[0]) * (I.cryst[1] / R.nc[1]) * (I.cryst[2] / R.nc[2]);
     int cap = E->tune.bucket_cap;
@@ -358,7 +367,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
             mloc[g] = pos;
         }
     }
-    std::vector<int> st = make_stencil(R.L, R.nc, R.rc, 0.05f);
+    std::vector<int> st = make_stencil(R.L, R.nc, R.rc, STENCIL_EPS);
     R.nst = (int)st.size();
 
     DevBuf &B = H.buf;
@@ -446,7 +455,7 @@ void launch_batched(gcmc_engine *E, int nsteps)
                        E->lay, 0, 1);
     if (E->timed_kernel) (void)hipEventRecord(E->evk0, E->stream);
     hipLaunchKernelGGL(k_batched<S>, dim3(E->nwg), dim3(WG), lds, E->stream, E->d_rep, E->d_state, R,
-                       E->d_sched, E->lay, E->spin_limit);
+                       E->d_sched, E->lay, E->spin_limit, E->xm);
     if (E->timed_kernel) (void)hipEventRecord(E->evk1, E->stream);
 }
 
@@ -464,6 +473,43 @@ void launch_plain(gcmc_engine *E, int nsteps)
         const int n = std::min(nsteps, std::max(rem, 0));
         for (int s = 0; s < n; ++s) launch_plain_step<S>(E, r);
     }
+}
+
+// Which XCC ids the device's workgroups run on, and a round-robin assignment
+// of replicas to them (each replica is then served by one XCD only).
+int build_xcd_map(gcmc_engine *E, int ncu)
+{
+    const int nprobe = std::max(64, 8 * ncu);
+    int *d = nullptr;
+    HIPX(hipMalloc(&d, nprobe * sizeof(int)));
+    hipLaunchKernelGGL(k_xcc_probe, dim3(nprobe), dim3(64), 0, E->stream, d);
+    std::vector<int> ids(nprobe);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(ids.data(), d, nprobe * sizeof(int), hipMemcpyDeviceToHost, E->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(E->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(GCMC_E_HIP, std::string("XCC probe: ") + hipGetErrorString(e));
+    XcdMap &m = E->xm;
+    for (int i = 0; i < 16; ++i) m.xcc_class[i] = -1;
+    m.nclass = 0;
+    for (int id : ids)
+        if (id >= 0 && id < 16 && m.xcc_class[id] < 0) m.xcc_class[id] = m.nclass++;
+    if (m.nclass == 0) return fail(GCMC_E_DEVICE, "XCC probe returned no ids");
+    const int R = (int)E->reps.size();
+    std::vector<int> cls(m.nclass + 1 + R, 0);
+    std::vector<std::vector<int>> members(m.nclass);
+    for (int r = 0; r < R; ++r) members[r % m.nclass].push_back(r);
+    int o = 0;
+    for (int c = 0; c < m.nclass; ++c) {
+        cls[c] = o;
+        for (int r : members[c]) cls[m.nclass + 1 + o++] = r;
+    }
+    cls[m.nclass] = o;
+    HIPX(hipMalloc(&E->d_cls, cls.size() * sizeof(int)));
+    HIPX(hipMemcpy(E->d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice));
+    m.cls_start = E->d_cls;
+    m.cls_reps = E->d_cls + m.nclass + 1;
+    return 0;
 }
 
 int read_states(gcmc_engine *E, std::vector<State> &st)
@@ -529,6 +575,7 @@ int gcmc_engine_destroy(gcmc_engine *E)
     if (E->d_rep) hipFree(E->d_rep);
     if (E->d_state) hipFree(E->d_state);
     if (E->d_sched) hipFree(E->d_sched);
+    if (E->d_cls) hipFree(E->d_cls);
     if (E->ev0) hipEventDestroy(E->ev0);
     if (E->ev1) hipEventDestroy(E->ev1);
     if (E->evk0) hipEventDestroy(E->evk0);
@@ -599,6 +646,8 @@ int gcmc_engine_upload(gcmc_engine *E)
     HIPX(hipGetDevice(&dev));
     HIPX(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     E->nwg = E->tune.persistent_wgs > 0 ? E->tune.persistent_wgs : 4 * std::max(ncu, 1);
+    rc = build_xcd_map(E, ncu);
+    if (rc) return rc;
     E->uploaded = true;
     return 0;
 }
@@ -686,6 +735,20 @@ int gcmc_engine_last_kernel_ms(gcmc_engine *E, double *ms)
 {
     if (!E || !ms) return fail(GCMC_E_ARG, "null argument");
     *ms = E->kernel_ms;
+    return 0;
+}
+
+int gcmc_engine_phase_ticks(gcmc_engine *E, int r, unsigned long long *out4)
+{
+    if (!E || !E->uploaded || !out4) return fail(GCMC_E_STATE, "engine not uploaded or null output");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HIPX(hipStreamSynchronize(E->stream));
+    State S;
+    HIPX(hipMemcpy(&S, E->d_state + r, sizeof(S), hipMemcpyDeviceToHost));
+    out4[0] = S.t_eval;
+    out4[1] = S.t_accept;
+    out4[2] = S.t_prep;
+    out4[3] = S.t_step;
     return 0;
 }
 
@@ -880,6 +943,38 @@ int gcmc_host_within(int n, const float *a, const float *b, const float *cryst, 
     const float r2max = r2_threshold(cutoff);
     for (int q = 0; q < n; ++q)
         out[q] = within(a[3 * q], a[3 * q + 1], a[3 * q + 2], b[3 * q], b[3 * q + 1], b[3 * q + 2], ax, r2max) ? 1 : 0;
+    return 0;
+}
+
+int gcmc_host_stencil_covers(int n, const float *centres, const float *groups, const float *cryst,
+                             const float *startxyz, float cutoff, float cell_size, unsigned char *out)
+{
+    if (n < 0 || (n > 0 && (!centres || !groups || !out)) || !cryst || !startxyz)
+        return fail(GCMC_E_ARG, "bad arguments");
+    int nc[3];
+    float inv_h[3], L[3], st[3];
+    for (int i = 0; i < 3; ++i) {
+        L[i] = cryst[i];
+        st[i] = startxyz[i];
+    }
+    cell_geometry(L, cell_size, nc, inv_h);
+    const std::vector<int> sten = make_stencil(L, nc, cutoff, STENCIL_EPS);
+    for (int q = 0; q < n; ++q) {
+        const int cc = cell_index(L, st, inv_h, nc, centres[3 * q], centres[3 * q + 1], centres[3 * q + 2]);
+        const int cg = cell_index(L, st, inv_h, nc, groups[3 * q], groups[3 * q + 1], groups[3 * q + 2]);
+        const int c0 = cc / (nc[1] * nc[2]), c1 = (cc / nc[2]) % nc[1], c2 = cc % nc[2];
+        out[q] = 0;
+        for (int code : sten) {
+            int i0 = c0 + (code & 255) - 128, i1 = c1 + ((code >> 8) & 255) - 128, i2 = c2 + ((code >> 16) & 255) - 128;
+            i0 = ((i0 % nc[0]) + nc[0]) % nc[0];
+            i1 = ((i1 % nc[1]) + nc[1]) % nc[1];
+            i2 = ((i2 % nc[2]) + nc[2]) % nc[2];
+            if ((i0 * nc[1] + i1) * nc[2] + i2 == cg) {
+                out[q] = 1;
+                break;
+            }
+        }
+    }
     return 0;
 }
 

@@ -315,7 +315,17 @@ __device__ inline bool tracing(const Rep &C, int step, int &row)
 // ---- one trial evaluation ----------------------------------------------------
 // item < k: trial item (insert/delete) or E_old (translate/rotate);
 // item >= k: E_new of trial item-k (translate/rotate).
-template <bool STAGED>
+// Replica data is read and written by workgroups of one XCD (batched form)
+// or by consecutive kernels (plain form).  Counters shared by workgroups use
+// L2 atomics of that XCD (LOCAL) or device-wide atomics.
+template <bool LOCAL, class T>
+__device__ inline void count_add(T *p, T v)
+{
+    if (LOCAL) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <bool STAGED, bool LOCAL>
 __device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -414,11 +424,11 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
     if (tid == 0) {
         C.E[phase * k + t] = E;
         State &W = const_cast<State &>(S);
-        atomicAdd(&W.n_eval, 1ull);
-        atomicAdd(&W.n_pair, (unsigned long long)nat * (unsigned long long)nf);
-        atomicAdd(&W.n_atom, (unsigned long long)nat);
-        atomicAdd(&W.n_group, (unsigned long long)nin);
-        atomicAdd(&W.n_fatom, (unsigned long long)nf);
+        count_add<LOCAL>(&W.n_eval, 1ull);
+        count_add<LOCAL>(&W.n_pair, (unsigned long long)nat * (unsigned long long)nf);
+        count_add<LOCAL>(&W.n_atom, (unsigned long long)nat);
+        count_add<LOCAL>(&W.n_group, (unsigned long long)nin);
+        count_add<LOCAL>(&W.n_fatom, (unsigned long long)nf);
         int row;
         if (tracing(C, step, row) && t < C.tr.max_k) {
             const int q = row * C.tr.max_k + t;
@@ -795,26 +805,39 @@ __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
     __syncthreads();
 }
 
+// XCD-local ordering.  Every workgroup touching a replica in the batched form
+// runs on the same XCD, whose L2 is shared: a release waits until this
+// thread's memory operations reached L2; an acquire drops the CU's L1 so
+// later loads read L2.  (Agent-scope fences would write back and invalidate
+// the whole L2 of the XCD on every claim.)
+__device__ inline void xcd_release() { __builtin_amdgcn_s_waitcnt(0); }
+__device__ inline void xcd_acquire() { asm volatile("buffer_inv sc0" ::: "memory"); }
+
 // advance past empty moves and publish the next move with work (batched form)
 __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L)
 {
     while (true) {
         if (S.step >= S.end) {
+            xcd_release();
+            __syncthreads();
             if (threadIdx.x == 0) {
-                __threadfence();
-                __hip_atomic_store(&S.claim, claim_word(S.step, 0, 0), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_fetch_add(&sch->finished, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&S.claim, claim_word(S.step, 0, 0), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&sch->finished, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             break;
         }
         prep_step(C, S, L);
         if (S.nitems > 0) {
-            __threadfence();
+            if (threadIdx.x == 0) {
+                S.t_pub = wall_clock64();
+                __hip_atomic_store(&S.done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            xcd_release();
             __syncthreads();
             if (threadIdx.x == 0)
-                __hip_atomic_store(&S.claim, claim_word(S.step, S.nitems, 0), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&S.claim, claim_word(S.step, S.nitems, 0), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
             break;
         }
         accept_step(C, S, L);
@@ -860,7 +883,7 @@ __global__ __launch_bounds__(WG) void k_eval_plain(const Rep *reps, const State 
     const State &S = st[r];
     if (S.step >= S.end || (int)blockIdx.x >= S.nitems) return;
     const Lds L = carve(smem, lay);
-    eval_item<STAGED>(reps[r], S, blockIdx.x, L);
+    eval_item<STAGED, false>(reps[r], S, blockIdx.x, L);
 }
 
 template <bool STAGED>
@@ -874,17 +897,35 @@ __global__ __launch_bounds__(WG) void k_accept_plain(const Rep *reps, State *st,
     if (S.step < S.end) prep_step(reps[r], S, L);
 }
 
-// persistent form: workgroups claim evaluations from any replica; the
-// workgroup completing a replica's last evaluation runs its accept and prep
-// phases and publishes the next move.  Exits when every replica finished or
-// after spin_limit idle polls (then sch->abort is set).
+// Which replicas a workgroup serves: the XCD it runs on (XCC_ID hardware
+// register) selects a class; class c owns replicas cls_reps[cls_start[c] ..
+// cls_start[c+1]).  xcc_class maps XCC ids to classes (-1: none).
+struct XcdMap {
+    int nclass;
+    int xcc_class[16];
+    const int *cls_start;
+    const int *cls_reps;
+};
+
+__device__ inline int my_xcc() { return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (15 << 11)) & 0xf); }
+
+// persistent form: workgroups of an XCD claim evaluations from that XCD's
+// replicas; the workgroup completing a replica's last evaluation runs its
+// accept and prep phases and publishes the next move.  Exits when every
+// replica finished, or when no replica advanced during spin_limit polls
+// (then sch->abort is set).
 template <bool STAGED>
 __global__ __launch_bounds__(WG) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
-                                               LdsLayout lay, int spin_limit)
+                                               LdsLayout lay, int spin_limit, XcdMap xm)
 {
     extern __shared__ __align__(16) char smem[];
     const Lds L = carve(smem, lay);
     const int tid = threadIdx.x;
+    const int cls = xm.xcc_class[my_xcc()];
+    if (cls < 0) return;
+    const int c0 = xm.cls_start[cls], nc = xm.cls_start[cls + 1] - c0;
+    if (nc <= 0) return;
+    const int rot = (int)(blockIdx.x / (unsigned)xm.nclass) % nc;
     int idle = 0, seen = -1;
     while (true) {
         if (tid == 0) {
@@ -898,13 +939,14 @@ __global__ __launch_bounds__(WG) void k_batched(const Rep *reps, State *st, int 
                 __hip_atomic_load(&sch->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
                 found = -2;
             } else {
-                for (int j = 0; j < R; ++j) {
-                    const int r = (int)((blockIdx.x + (unsigned)j) % (unsigned)R);
-                    const unsigned long long w =
-                        __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                for (int j = 0; j < nc; ++j) {
+                    const int r = xm.cls_reps[c0 + (rot + j) % nc];
+                    // an L2 read-modify-write sees the latest claim word of this XCD
+                    const unsigned long long w = __hip_atomic_fetch_or(&st[r].claim, 0ull, __ATOMIC_RELAXED,
+                                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
                     if ((w & 0xFFFFFull) >= ((w >> 20) & 0xFFFFFull)) continue;
-                    const unsigned long long old = __hip_atomic_fetch_add(
-                        &st[r].claim, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                    const unsigned long long old = __hip_atomic_fetch_add(&st[r].claim, 1ull, __ATOMIC_RELAXED,
+                                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
                     const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
                     if (c < n) {
                         found = r;
@@ -930,25 +972,41 @@ __global__ __launch_bounds__(WG) void k_batched(const Rep *reps, State *st, int 
             __builtin_amdgcn_s_sleep(8);
             continue;
         }
-        __threadfence();
+        xcd_acquire();
         const Rep &C = reps[found];
         State &S = st[found];
-        eval_item<STAGED>(C, S, item, L);
-        __threadfence();
+        const unsigned long long te0 = wall_clock64();
+        eval_item<STAGED, true>(C, S, item, L);
+        if (tid == 0) count_add<true>(&S.t_eval, wall_clock64() - te0);
+        xcd_release();
         __syncthreads();
         if (tid == 0) {
-            const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             L.misc[30] = (old == S.nitems - 1) ? 1 : 0;
         }
         __syncthreads();
         if (L.misc[30]) {
-            __threadfence();
+            xcd_acquire();
+            const unsigned long long ta0 = wall_clock64();
+            if (tid == 0) S.t_step += ta0 - S.t_pub;
             accept_step(C, S, L);
+            const unsigned long long ta1 = wall_clock64();
             advance_and_publish(C, S, sch, L);
-            if (tid == 0) __hip_atomic_fetch_add(&sch->progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tid == 0) {
+                const unsigned long long ta2 = wall_clock64();
+                S.t_accept += ta1 - ta0;
+                S.t_prep += ta2 - ta1;
+                __hip_atomic_fetch_add(&sch->progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
         __syncthreads();
     }
+}
+
+// XCC id of each workgroup (engine set-up: which XCDs exist)
+__global__ void k_xcc_probe(int *out)
+{
+    if (threadIdx.x == 0) out[blockIdx.x] = my_xcc();
 }
 
 // energies of given poses (tests and the adjoint's forward)

@@ -95,6 +95,10 @@ struct State {
     // work counters: evaluations, fragment-environment pairs, included
     // environment atoms and groups, fragment atoms (summed over evaluations)
     unsigned long long n_eval, n_pair, n_atom, n_group, n_fatom;
+    // 100 MHz wall-clock ticks spent in evaluations, accept and prep phases,
+    // and between a move's publication and its last evaluation finishing
+    unsigned long long t_eval, t_accept, t_prep, t_step;
+    unsigned long long t_pub;
 };
 
 struct Sched {

@@ -72,10 +72,12 @@ def lib():
         L.gcmc_engine_last_run_ms.argtypes = [P, P]
         L.gcmc_engine_last_kernel_ms.argtypes = [P, P]
         L.gcmc_engine_counters.argtypes = [P, I, P]
+        L.gcmc_engine_phase_ticks.argtypes = [P, I, P]
         L.gcmc_engine_set_trace.argtypes = [P, I, I, I]
         L.gcmc_engine_get_trace.argtypes = [P, I, ctypes.POINTER(CTrace)]
         L.gcmc_engine_eval_poses.argtypes = [P, I, I, I, P, P, P, P, P, P]
         L.gcmc_host_within.argtypes = [I, P, P, P, ctypes.c_float, P]
+        L.gcmc_host_stencil_covers.argtypes = [I, P, P, P, P, ctypes.c_float, ctypes.c_float, P]
         L.runGCMC_hip.argtypes = [P, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -105,6 +107,18 @@ def host_within(a, b, cryst, cutoff):
     c = np.ascontiguousarray(cryst, np.float32)
     out = np.zeros(len(a), np.uint8)
     _check(lib().gcmc_host_within(len(a), _p(a), _p(b), _p(c), float(cutoff), _p(out)))
+    return out.astype(bool)
+
+
+def host_stencil_covers(centres, groups, cryst, startxyz, cutoff, cell_size=0.0):
+    """Whether each group's cell is scanned for the paired centre (host check)."""
+    a = np.ascontiguousarray(centres, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(groups, np.float32).reshape(-1, 3)
+    c = np.ascontiguousarray(cryst, np.float32)
+    st = np.ascontiguousarray(startxyz, np.float32)
+    out = np.zeros(len(a), np.uint8)
+    _check(lib().gcmc_host_stencil_covers(len(a), _p(a), _p(b), _p(c), _p(st), float(cutoff),
+                                          float(cell_size), _p(out)))
     return out.astype(bool)
 
 
@@ -175,6 +189,12 @@ class Engine:
         out = np.zeros(5, np.uint64)
         _check(lib().gcmc_engine_counters(self._h, r, _p(out)))
         return dict(zip(('evals', 'pairs', 'atoms', 'groups', 'frag_atoms'), (int(x) for x in out)))
+
+    def phase_seconds(self, r=0):
+        """Device time of a replica by phase (batched form), seconds."""
+        out = np.zeros(4, np.uint64)
+        _check(lib().gcmc_engine_phase_ticks(self._h, r, _p(out)))
+        return dict(zip(('eval', 'accept', 'prep', 'step'), (float(x) * 1e-8 for x in out)))
 
     def step(self, r=0):
         return _check(lib().gcmc_engine_step(self._h, r))

@@ -1,0 +1,126 @@
+"""Host-side logic of the engine (no GPU): the C ABI exports, the division-free
+FP32 group test against the oracle's literal formula, the cell-stencil
+superset property, set-up restatements and argument checking."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pygcmc_amd import engine, synth
+from pygcmc_amd.values import AtomArray_dtype, Atom_dtype, Info_dtype, Residue_dtype
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, 'include', 'pygcmc_hip.h')).read()
+    names = re.findall(r'^\s*(?:int|const char \*|void)\s+\*?(\w+)\(', hdr, re.M)
+    assert 'runGCMC_hip' in names and len(names) >= 15
+    lib = ctypes.CDLL(engine.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_cpython_module_signature():
+    from pygcmc_amd import gpu
+    assert callable(gpu.runGCMC)
+    with pytest.raises(TypeError):
+        gpu.runGCMC(1, 2, 3)
+
+
+def test_cpython_module_checks_arrays():
+    from pygcmc_amd import gpu
+    s = synth.water_sphere(mcsteps=10)
+    with pytest.raises(TypeError):   # wrong item size
+        gpu.runGCMC(s['info'], s['res'], s['res'], s['atoms'], s['grid'], s['ff'], s['moves'])
+    with pytest.raises(ValueError):  # too few residues for the InfoStruct
+        gpu.runGCMC(s['info'], s['frag'], s['res'][:10], s['atoms'], s['grid'], s['ff'], s['moves'])
+
+
+def _boundary_points(rng, L, rc, n):
+    """Pairs concentrated where the image shift or the cutoff test flips."""
+    L = np.asarray(L, np.float64)
+    a = rng.uniform(-0.3, 1.3, (n, 3)) * L
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    kind = rng.integers(0, 3, n)
+    r = np.where(kind == 0, rc * (1 + rng.normal(0, 1e-6, n)), rng.uniform(0, 1.6 * L.max(), n))
+    b = a + d * r[:, None]
+    # put some axis separations right at half a box
+    m = kind == 1
+    b[m, 0] = a[m, 0] + L[0] * 0.5 * (1 + rng.normal(0, 1e-6, m.sum()))
+    return a.astype(np.float32), b.astype(np.float32)
+
+
+@pytest.mark.parametrize('L', [[40.0, 40.0, 40.0], [132.126, 132.213, 122.07], [54.31, 61.7, 80.0]])
+def test_division_free_group_test_matches_literal(L):
+    rng = np.random.default_rng(17)
+    rc = 15.0
+    a, b = _boundary_points(rng, L, rc, 20000)
+    got = engine.host_within(a, b, L, rc)
+    ref = np.array([O.distanceP32(a[i], b[i], L) <= np.float32(rc) for i in range(len(a))])
+    assert np.array_equal(got, ref)
+    assert 0 < ref.sum() < len(ref)
+
+
+@pytest.mark.parametrize('cell', [0.0, 3.5, 5.0, 7.5, 16.0])
+def test_stencil_covers_every_included_group(cell):
+    rng = np.random.default_rng(23)
+    L = [54.31, 61.7, 80.0]
+    start = [-1.25, 0.5, 3.0]
+    rc = 15.0
+    a, b = _boundary_points(rng, L, rc, 20000)
+    inc = engine.host_within(a, b, L, rc)
+    cov = engine.host_stencil_covers(a, b, L, start, rc, cell)
+    assert cov[inc].all()
+    assert (~cov).any()  # the stencil really prunes
+
+
+def test_stencil_small_box_wraps_once():
+    rng = np.random.default_rng(4)
+    L = [30.0, 30.0, 30.0]
+    a, b = _boundary_points(rng, L, 15.0, 5000)
+    cov = engine.host_stencil_covers(a, b, L, [0, 0, 0], 15.0, 5.0)
+    inc = engine.host_within(a, b, L, 15.0)
+    assert cov[inc].all()
+
+
+def test_engine_rejects_bad_systems():
+    if not engine.os.path.exists(engine.LIB_PATH):
+        pytest.skip('library not built')
+    s = synth.water_sphere(mcsteps=5)
+    bad = O.copy_system(s)
+    bad['frag'][0]['confBias'] = 5000
+    try:
+        eng = engine.Engine()
+    except RuntimeError as e:  # no GPU in this container: creation itself reports it
+        assert 'no HIP device' in str(e)
+        return
+    with pytest.raises(RuntimeError, match='confBias'):
+        eng.add_system(bad)
+    eng.close()
+
+
+def test_dtypes_match_struct_sizes():
+    assert (Atom_dtype.itemsize, AtomArray_dtype.itemsize, Info_dtype.itemsize,
+            Residue_dtype.itemsize) == (20, 436, 76, 20)
+
+
+def test_replica_sizes_and_boxes():
+    sizes = synth.replica_sizes(64)
+    assert len(sizes) == 64 and sizes.min() >= 5000 and sizes.max() <= 60000
+    assert len(set(sizes.tolist())) == 64
+
+
+def test_fragment_templates_have_config_atom_counts():
+    counts = {name: len(b()[1]) for name, b, _, _ in synth.FRAGMENTS}
+    assert counts == {'BENX': 12, 'PRPX': 11, 'MEOH': 6, 'FORM': 6, 'ALDX': 7, 'IMIA': 9,
+                      'MAMY': 8, 'ACEY': 7}
+    rng = np.random.default_rng(0)
+    for name, b, net, _ in synth.FRAGMENTS:
+        q = synth.fragment_charges(rng, len(b()[1]), net)
+        assert abs(q.sum() - net) < 1e-12
