@@ -84,9 +84,10 @@ typedef struct gcmc_engine gcmc_engine;
 
 typedef struct gcmc_tuning {
     float cell_size;     /* target edge (A) of the group-centre cell list; 0 = 5 A */
-    int persistent_wgs;  /* workgroups of the batched kernel; 0 = 4 per CU */
+    int persistent_wgs;  /* workgroups of the batched kernel; 0 = per CU by wg_size */
     int bucket_cap;      /* mobile groups per cell before overflow; 0 = auto */
     int pair_lds;        /* 1 = stage the LJ pair table in LDS when it fits */
+    int wg_size;         /* threads per workgroup: 256, 512 or 1024; 0 = 256 */
 } gcmc_tuning;
 
 #define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */
@@ -127,6 +128,9 @@ int gcmc_engine_counters(gcmc_engine *e, int replica, unsigned long long *out5);
  * evaluations (summed over workgroups), [1] in accept phases, [2] in prep
  * phases, [3] from publishing a move to its last evaluation finishing. */
 int gcmc_engine_phase_ticks(gcmc_engine *e, int replica, unsigned long long *out4);
+/* Ticks inside evaluations (summed): [0] pose and adjacency, [1] stencil
+ * counts and scan, [2] candidate group tests, [3] pair passes. */
+int gcmc_engine_sub_ticks(gcmc_engine *e, int replica, unsigned long long *out4);
 
 /* ---- step trace (parity tests) --------------------------------------------
  * Per step s (relative to the step when tracing was enabled) and trial t:

@@ -137,6 +137,7 @@ struct gcmc_engine {
     hipGraphExec_t gexec = nullptr;
     int graph_nsteps = -1;
     int spin_limit = 1 << 22;  // polls without progress anywhere (~seconds)
+    int nt = 256;              // threads per workgroup
     XcdMap xm{};
     int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]
 };
@@ -291,13 +292,15 @@ int upload_rep(gcmc_engine *E, HostRep &H)
             if (H.atoms[a].type < 0 || H.atoms[a].type >= R.Y)
                 return fail(GCMC_E_ARG, "atom type outside the pair table columns");
     }
-    for (int s = 0; s < T; ++s)
+    for (int s = 0; s < T; ++s) {
+        const int nf = H.frag[s].num_atoms, base = H.res[H.frag[s].startRes].atomStart;
         for (int j = 0; j < H.frag[s].maxNum; ++j) {
             const residue &r = H.res[H.frag[s].startRes + j];
-            if (r.atomNum != H.frag[s].num_atoms || r.atomStart < 0 ||
+            if (r.atomNum != nf || r.atomStart != base + j * nf || r.atomStart < 0 ||
                 r.atomStart + r.atomNum > I.totalAtomNum)
-                return fail(GCMC_E_ARG, "fragment slot with a bad atom range");
+                return fail(GCMC_E_ARG, "fragment slots must hold contiguous atom ranges of num_atoms atoms");
         }
+    }
     for (int a = 0; a < I.totalAtomNum; ++a) {
         const Atom &A = H.atoms[a];
         atom[a] = make_float4(A.position[0], A.position[1], A.position[2], A.charge);
@@ -317,6 +320,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
         S.maxNum = F.maxNum;
         S.nf = F.num_atoms;
         S.conf = F.confBias;
+        S.atomBase = H.res[F.startRes].atomStart;
         double nbar = (double)I.cryst[0] * (double)I.cryst[1] * (double)I.cryst[2] *
                       (double)F.conc * 0.0006023;
         S.B = BETA * (double)F.muex + std::log(nbar);
@@ -340,6 +344,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     }
     for (int c = 0; c < R.ncell; ++c) fstart[c + 1] += fstart[c];
     std::vector<float4> frec(std::max(R.nfix, 1));
+    std::vector<int2> frng(std::max(R.nfix, 1));
     {
         std::vector<int> fill(fstart.begin(), fstart.end() - 1);
         for (int g = 0; g < R.nfix; ++g) {
@@ -347,13 +352,16 @@ int upload_rep(gcmc_engine *E, HostRep &H)
             int bits = g;
             float w;
             std::memcpy(&w, &bits, 4);
-            frec[fill[fcell[g]]++] = make_float4(r.position[0], r.position[1], r.position[2], w);
+            const int q = fill[fcell[g]]++;
+            frec[q] = make_float4(r.position[0], r.position[1], r.position[2], w);
+            frng[q] = make_int2(r.atomStart, r.atomNum);
         }
     }
     // mobile groups: buckets + overflow
     std::vector<int> mcnt(R.ncell + 1, 0);
     std::vector<int> mslot((size_t)R.ovf_base + std::max(nmob, 1), -1);
     std::vector<int> mloc(I.totalResNum, -1);
+    std::vector<float4> mrec(mslot.size(), make_float4(0.f, 0.f, 0.f, 0.f));
     for (int s = 0; s < T; ++s) {
         const AtomArray &F = H.frag[s];
         for (int j = 0; j < F.totalNum; ++j) {
@@ -365,6 +373,9 @@ int upload_rep(gcmc_engine *E, HostRep &H)
             else pos = R.ovf_base + mcnt[R.ncell]++;
             mslot[pos] = g;
             mloc[g] = pos;
+            float w;
+            std::memcpy(&w, &g, 4);
+            mrec[pos] = make_float4(r.position[0], r.position[1], r.position[2], w);
         }
     }
     std::vector<int> st = make_stencil(R.L, R.nc, R.rc, STENCIL_EPS);
@@ -381,6 +392,8 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     HIPX(put((Species **)&R.sp, B, sp.data(), sp.size()));
     HIPX(put((int **)&R.fix_start, B, fstart.data(), fstart.size()));
     HIPX(put((float4 **)&R.fix_rec, B, frec.data(), frec.size()));
+    HIPX(put((int2 **)&R.fix_rng, B, frng.data(), frng.size()));
+    HIPX(put(&R.mob_rec, B, mrec.data(), mrec.size()));
     HIPX(put(&R.mob_cnt, B, mcnt.data(), mcnt.size()));
     HIPX(put(&R.mob_slot, B, mslot.data(), mslot.size()));
     HIPX(put(&R.mob_loc, B, mloc.data(), mloc.size()));
@@ -404,13 +417,14 @@ int compute_layout(gcmc_engine *E)
     E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
     auto up = [](int x) { return (x + 15) & ~15; };
     LdsLayout &l = E->lay;
+    l.inc_cap = 2 * E->nt;
     int o = 0;
     l.scell = o; o = up(o + 4 * (nst + 2));
     l.sfix = o; o = up(o + 4 * (nst + 2));
     l.soff = o; o = up(o + 4 * (nst + 2));
-    l.incg = o; o = up(o + 4 * (INC_CAP + 1));
-    l.incs = o; o = up(o + 4 * (INC_CAP + 1));
-    l.incn = o; o = up(o + 4 * (INC_CAP + 1));
+    l.incg = o; o = up(o + 4 * (l.inc_cap + 1));
+    l.incs = o; o = up(o + 4 * (l.inc_cap + 1));
+    l.incn = o; o = up(o + 4 * (l.inc_cap + 1));
     l.frag = o; o = up(o + 16 * MAXF);
     l.frow = o; o = up(o + 4 * MAXF);
     if (E->staged) {
@@ -420,46 +434,49 @@ int compute_layout(gcmc_engine *E)
         l.pair = -1;
     }
     const int eval_end = o;
+    // accept and prep phases reuse the evaluation region
     int a = 0;
     l.members = a; a = up(a + 4 * E->maxk);
     l.w = a; a = up(a + 8 * E->maxk);
+    l.val = a; a = up(a + 8 * E->maxk);
     l.accl = a; a = up(a + 8 * E->maxk);
     l.J = 0;  // prep runs after accept in the same workgroup
     o = std::max(eval_end, a);
-    l.misc = o; o = up(o + 4 * 32);
-    l.dred = o; o = up(o + 8 * 2 * NWAVE);
+    l.spt = o; o = up(o + 16 * MAXT);
+    l.misc = o; o = up(o + 4 * MISC_INTS);
+    l.dred = o; o = up(o + 8 * MAXWAVE);
     l.total = o;
     if (l.total > 65536)
         return fail(GCMC_E_ARG, "LDS layout exceeds 64 KiB (stencil or confBias too large; raise cell_size)");
     return 0;
 }
 
-template <bool S>
+template <int NT, bool S>
 void launch_plain_step(gcmc_engine *E, int r)
 {
     const size_t lds = E->lay.total;
-    hipLaunchKernelGGL(k_eval_plain<S>, dim3(2 * E->reps[r]->maxk), dim3(WG), lds, E->stream,
-                       E->d_rep, E->d_state, r, E->lay);
-    hipLaunchKernelGGL(k_accept_plain<S>, dim3(1), dim3(WG), lds, E->stream, E->d_rep, E->d_state, r,
+    hipLaunchKernelGGL((k_eval_plain<NT, S>), dim3(2 * E->reps[r]->maxk), dim3(NT), lds, E->stream, E->d_rep,
+                       E->d_state, r, E->lay);
+    hipLaunchKernelGGL((k_accept_plain<NT, S>), dim3(1), dim3(NT), lds, E->stream, E->d_rep, E->d_state, r,
                        E->lay);
 }
 
-template <bool S>
+template <int NT, bool S>
 void launch_batched(gcmc_engine *E, int nsteps)
 {
     const int R = (int)E->reps.size();
     const size_t lds = E->lay.total;
     hipLaunchKernelGGL(k_begin, dim3((R + 255) / 256), dim3(256), 0, E->stream, E->d_state, E->d_rep, 0, R,
                        nsteps, E->d_sched);
-    hipLaunchKernelGGL(k_start<S>, dim3(R), dim3(WG), lds, E->stream, E->d_rep, E->d_state, E->d_sched,
+    hipLaunchKernelGGL((k_start<NT, S>), dim3(R), dim3(NT), lds, E->stream, E->d_rep, E->d_state, E->d_sched,
                        E->lay, 0, 1);
     if (E->timed_kernel) (void)hipEventRecord(E->evk0, E->stream);
-    hipLaunchKernelGGL(k_batched<S>, dim3(E->nwg), dim3(WG), lds, E->stream, E->d_rep, E->d_state, R,
+    hipLaunchKernelGGL((k_batched<NT, S>), dim3(E->nwg), dim3(NT), lds, E->stream, E->d_rep, E->d_state, R,
                        E->d_sched, E->lay, E->spin_limit, E->xm);
     if (E->timed_kernel) (void)hipEventRecord(E->evk1, E->stream);
 }
 
-template <bool S>
+template <int NT, bool S>
 void launch_plain(gcmc_engine *E, int nsteps)
 {
     const size_t lds = E->lay.total;
@@ -467,13 +484,39 @@ void launch_plain(gcmc_engine *E, int nsteps)
     for (int r = 0; r < R; ++r) {
         hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, E->stream, E->d_state, E->d_rep, r, 1, nsteps,
                            E->d_sched);
-        hipLaunchKernelGGL(k_start<S>, dim3(1), dim3(WG), lds, E->stream, E->d_rep, E->d_state, E->d_sched,
-                           E->lay, r, 0);
+        hipLaunchKernelGGL((k_start<NT, S>), dim3(1), dim3(NT), lds, E->stream, E->d_rep, E->d_state,
+                           E->d_sched, E->lay, r, 0);
         const int rem = E->reps[r]->rep.nmoves - E->reps[r]->host_step;
         const int n = std::min(nsteps, std::max(rem, 0));
-        for (int s = 0; s < n; ++s) launch_plain_step<S>(E, r);
+        for (int s = 0; s < n; ++s) launch_plain_step<NT, S>(E, r);
     }
 }
+
+// dispatch on (workgroup size, pair table in LDS)
+template <template <int, bool> class F, class... A>
+void dispatch(gcmc_engine *E, A... args)
+{
+    if (E->nt == 1024) E->staged ? F<1024, true>::run(E, args...) : F<1024, false>::run(E, args...);
+    else if (E->nt == 512) E->staged ? F<512, true>::run(E, args...) : F<512, false>::run(E, args...);
+    else E->staged ? F<256, true>::run(E, args...) : F<256, false>::run(E, args...);
+}
+template <int NT, bool S>
+struct RunPlain {
+    static void run(gcmc_engine *E, int n) { launch_plain<NT, S>(E, n); }
+};
+template <int NT, bool S>
+struct RunBatched {
+    static void run(gcmc_engine *E, int n) { launch_batched<NT, S>(E, n); }
+};
+template <int NT, bool S>
+struct RunPoses {
+    static void run(gcmc_engine *E, int npose, int r, int species, const float *dc, const float *dp, const int *dx,
+                    double *de, int *dn, uint32_t *dh)
+    {
+        hipLaunchKernelGGL((k_eval_poses<NT, S>), dim3(npose), dim3(NT), E->lay.total, E->stream, E->d_rep, r,
+                           species, dc, dp, dx, de, dn, dh, E->lay);
+    }
+};
 
 // Which XCC ids the device's workgroups run on, and a round-robin assignment
 // of replicas to them (each replica is then served by one XCD only).
@@ -624,6 +667,7 @@ int gcmc_engine_upload(gcmc_engine *E)
         int rc = upload_rep(E, *H);
         if (rc) return rc;
     }
+    E->nt = (E->tune.wg_size == 512 || E->tune.wg_size == 1024) ? E->tune.wg_size : 256;
     int rc = compute_layout(E);
     if (rc) return rc;
     const int R = (int)E->reps.size();
@@ -645,7 +689,7 @@ int gcmc_engine_upload(gcmc_engine *E)
     int dev = 0, ncu = 0;
     HIPX(hipGetDevice(&dev));
     HIPX(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    E->nwg = E->tune.persistent_wgs > 0 ? E->tune.persistent_wgs : 4 * std::max(ncu, 1);
+    E->nwg = E->tune.persistent_wgs > 0 ? E->tune.persistent_wgs : (1024 / E->nt) * std::max(ncu, 1);
     rc = build_xcd_map(E, ncu);
     if (rc) return rc;
     E->uploaded = true;
@@ -659,12 +703,10 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
     if (nsteps == 0) return 0;
     HIPX(hipEventRecord(E->ev0, E->stream));
     if (mode == GCMC_MODE_PLAIN) {
-        if (E->staged) launch_plain<true>(E, nsteps);
-        else launch_plain<false>(E, nsteps);
+        dispatch<RunPlain>(E, nsteps);
     } else if (mode == GCMC_MODE_BATCHED) {
         E->timed_kernel = true;
-        if (E->staged) launch_batched<true>(E, nsteps);
-        else launch_batched<false>(E, nsteps);
+        dispatch<RunBatched>(E, nsteps);
         E->timed_kernel = false;
     } else if (mode == GCMC_MODE_GRAPH) {
         if (!E->gexec || E->graph_nsteps != nsteps) {
@@ -674,8 +716,7 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
                 E->gexec = nullptr;
             }
             HIPX(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-            if (E->staged) launch_batched<true>(E, nsteps);
-            else launch_batched<false>(E, nsteps);
+            dispatch<RunBatched>(E, nsteps);
             HIPX(hipStreamEndCapture(E->stream, &E->graph));
             HIPX(hipGraphInstantiate(&E->gexec, E->graph, nullptr, nullptr, 0));
             E->graph_nsteps = nsteps;
@@ -735,6 +776,17 @@ int gcmc_engine_last_kernel_ms(gcmc_engine *E, double *ms)
 {
     if (!E || !ms) return fail(GCMC_E_ARG, "null argument");
     *ms = E->kernel_ms;
+    return 0;
+}
+
+int gcmc_engine_sub_ticks(gcmc_engine *E, int r, unsigned long long *out4)
+{
+    if (!E || !E->uploaded || !out4) return fail(GCMC_E_STATE, "engine not uploaded or null output");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HIPX(hipStreamSynchronize(E->stream));
+    State S;
+    HIPX(hipMemcpy(&S, E->d_state + r, sizeof(S), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4; ++i) out4[i] = S.t_sub[i];
     return 0;
 }
 
@@ -912,12 +964,7 @@ int gcmc_engine_eval_poses(gcmc_engine *E, int r, int species, int npose, const 
     if (e == hipSuccess) e = B.alloc(&dn, npose);
     if (e == hipSuccess) e = B.alloc(&dh, npose);
     if (e == hipSuccess) {
-        if (E->staged)
-            hipLaunchKernelGGL(k_eval_poses<true>, dim3(npose), dim3(WG), E->lay.total, E->stream, E->d_rep, r,
-                               species, dc, dp, dx, de, dn, dh, E->lay);
-        else
-            hipLaunchKernelGGL(k_eval_poses<false>, dim3(npose), dim3(WG), E->lay.total, E->stream, E->d_rep,
-                               r, species, dc, dp, dx, de, dn, dh, E->lay);
+        dispatch<RunPoses>(E, npose, r, species, (const float *)dc, (const float *)dp, (const int *)dx, de, dn, dh);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(E->stream);

@@ -1,8 +1,8 @@
 // engine_kernels.h -- device phases of one GCMC move and the kernels that
 // schedule them.
 //
-// A move of one replica is three phases, each run by one 256-thread
-// workgroup:
+// A move of one replica is three phases, each run by one workgroup of NT
+// threads (NT = 256, 512 or 1024, a tuning choice):
 //   prep    decode moveArray[step]; pick slots (partial Fisher-Yates);
 //           trial centres                      (gcmc_move_*.h host side)
 //   item    one trial evaluation: build the pose, scan the cell list,
@@ -10,27 +10,44 @@
 //   accept  grouping of trials, selection, Metropolis in FP64, commit
 //           (move_add/del/trn/rot host logic, Gupdate*)
 // The plain form launches item and accept kernels per move; the batched form
-// runs every replica in one persistent kernel where workgroups claim items
-// from any replica and the workgroup finishing a replica's last item runs its
-// accept and prep phases (no grid-wide barrier, replicas advance
-// independently).
+// runs every replica in one persistent kernel where workgroups of an XCD
+// claim items of that XCD's replicas and the workgroup finishing a replica's
+// last item runs its accept and prep phases (no grid-wide barrier, replicas
+// advance independently).
 #pragma once
 
 #include "engine_types.h"
 
 namespace gcmc {
 
+// scratch ints of a workgroup (Lds::misc)
+enum Misc : int {
+    M_SCAN = 0,      // [0, NW]   scan partials, total at NW
+    M_WCNT = 20,     // [20, 36)  included groups per wave
+    M_NINC = 36,     //           included groups buffered
+    M_NIN = 40,      // [40, 56)  per-wave included count
+    M_HASH = 56,     // [56, 72)  per-wave set hash
+    M_ACC = 80,      // accept: decision / count of accepted
+    M_NG = 81,       //         groups
+    M_N = 82,        //         species count after the move
+    M_FOUND = 88,    // batched: replica claimed (-1 none, -2 exit)
+    M_ITEM = 89,     //          item claimed
+    M_LAST = 90,     //          this workgroup finished the move's last item
+};
+
 struct Lds {
     int *scell, *sfix, *soff, *incg, *incs, *incn;
     float4 *frag;
     int *frow;
     float2 *pair;
+    int4 *spt;
     int *misc;
     double *dred;
     int *members;
-    double *w;
+    double *w, *val;
     int2 *accl;
     int *J;
+    int inc_cap;
 };
 
 __device__ inline Lds carve(char *b, const LdsLayout &l)
@@ -45,12 +62,15 @@ __device__ inline Lds carve(char *b, const LdsLayout &l)
     s.frag = (float4 *)(b + l.frag);
     s.frow = (int *)(b + l.frow);
     s.pair = (float2 *)(b + (l.pair >= 0 ? l.pair : 0));
+    s.spt = (int4 *)(b + l.spt);
     s.misc = (int *)(b + l.misc);
     s.dred = (double *)(b + l.dred);
     s.members = (int *)(b + l.members);
     s.w = (double *)(b + l.w);
+    s.val = (double *)(b + l.val);
     s.accl = (int2 *)(b + l.accl);
     s.J = (int *)(b + l.J);
+    s.inc_cap = l.inc_cap;
     return s;
 }
 
@@ -61,10 +81,12 @@ __device__ inline void wave_sync()
 }
 
 // exclusive scan of a[0..n) in LDS, a[n] = total; returns total
+template <int NT>
 __device__ inline int wg_scan(int *a, int n, int *misc)
 {
+    constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int per = (n + WG - 1) / WG;
+    const int per = (n + NT - 1) / NT;
     const int b = tid * per;
     const int e = min(n, b + per);
     int sum = 0;
@@ -75,25 +97,25 @@ __device__ inline int wg_scan(int *a, int n, int *misc)
         int y = __shfl_up(x, off, 64);
         if (lane >= off) x += y;
     }
-    if (lane == 63) misc[wave] = x;
+    if (lane == 63) misc[M_SCAN + wave] = x;
     __syncthreads();
     if (tid == 0) {
         int t = 0;
-        for (int w = 0; w < NWAVE; ++w) {
-            int v = misc[w];
-            misc[w] = t;
+        for (int w = 0; w < NW; ++w) {
+            int v = misc[M_SCAN + w];
+            misc[M_SCAN + w] = t;
             t += v;
         }
-        misc[NWAVE] = t;
+        misc[M_SCAN + NW] = t;
     }
     __syncthreads();
-    int base = misc[wave] + x - sum;
+    int base = misc[M_SCAN + wave] + x - sum;
     for (int i = b; i < e; ++i) {
         int v = a[i];
         a[i] = base;
         base += v;
     }
-    const int total = misc[NWAVE];
+    const int total = misc[M_SCAN + NW];
     if (tid == 0) a[n] = total;
     __syncthreads();
     return total;
@@ -124,14 +146,14 @@ __device__ inline float two_diff(float a, float b, float &e)
 // ---- pair energies of the buffered included groups ------------------------
 // lane <-> environment atom (float4 load, coalesced within a group), loop
 // over the fragment atoms held in LDS; FP32 terms, TwoSum into (acc, cmp).
-template <bool STAGED>
+template <int NT, bool STAGED>
 __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
 {
-    const int A = wg_scan(L.incn, n, L.misc);
+    const int A = wg_scan<NT>(L.incn, n, L.misc);
     const float2 *PT = STAGED ? L.pair : C.pair;
     const float Lx = C.L[0], Ly = C.L[1], Lz = C.L[2];
     const float ix = C.invL[0], iy = C.invL[1], iz = C.invL[2];
-    for (int t = threadIdx.x; t < A; t += WG) {
+    for (int t = threadIdx.x; t < A; t += NT) {
         const int g = bsearch_le(L.incn, n, t);
         const int a = L.incs[g] + (t - L.incn[g]);
         const float4 at = C.atom[a];
@@ -170,30 +192,44 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
     return A;
 }
 
+// species table in LDS (mobile group id -> atom range)
+__device__ inline void load_species_table(const Rep &C, const Lds &L)
+{
+    for (int s = threadIdx.x; s < C.T; s += blockDim.x) {
+        const Species &F = C.sp[s];
+        L.spt[s] = make_int4(F.startRes, F.maxNum, F.atomBase, F.nf);
+    }
+}
+
 // ---- calcEnergy over the cell list -----------------------------------------
 // Candidates: fixed and mobile groups of the stencil cells around the test
 // centre plus the overflow bucket; kept when gid != exclude and the FP32
-// distanceP test passes (gcmc_energy.h:152-191).  Thread 0 receives results.
-template <bool STAGED>
+// distanceP test passes (gcmc_energy.h:152-191).  Needs L.spt loaded.
+// Thread 0 receives the results.
+template <int NT, bool STAGED>
 __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, int nf, int exclude,
-                                   const Lds &L, double *outE, int *outN, uint32_t *outH, int *outA)
+                                   const Lds &L, double *outE, int *outN, uint32_t *outH, int *outA,
+                                   unsigned long long *ticks = nullptr)
 {
+    unsigned long long tk0 = wall_clock64(), tpair = 0;
+    constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int cc = cell_index(C.L, C.start, C.inv_h, C.nc, cx, cy, cz);
     const int c2 = cc % C.nc[2];
     const int c1 = (cc / C.nc[2]) % C.nc[1];
     const int c0 = cc / (C.nc[1] * C.nc[2]);
     const int ne = C.nst + 1;
-    for (int e = tid; e < ne; e += WG) {
+    for (int e = tid; e < ne; e += NT) {
         int cell, fc, mc;
         if (e < C.nst) {
             const int code = C.stencil[e];
+            // stencil offsets span at most one box per axis: wrap by one add
             int i0 = c0 + (code & 255) - 128;
             int i1 = c1 + ((code >> 8) & 255) - 128;
             int i2 = c2 + ((code >> 16) & 255) - 128;
-            i0 = ((i0 % C.nc[0]) + C.nc[0]) % C.nc[0];
-            i1 = ((i1 % C.nc[1]) + C.nc[1]) % C.nc[1];
-            i2 = ((i2 % C.nc[2]) + C.nc[2]) % C.nc[2];
+            i0 += i0 < 0 ? C.nc[0] : (i0 >= C.nc[0] ? -C.nc[0] : 0);
+            i1 += i1 < 0 ? C.nc[1] : (i1 >= C.nc[1] ? -C.nc[1] : 0);
+            i2 += i2 < 0 ? C.nc[2] : (i2 >= C.nc[2] ? -C.nc[2] : 0);
             cell = (i0 * C.nc[1] + i1) * C.nc[2] + i2;
             fc = C.fix_start[cell + 1] - C.fix_start[cell];
             mc = C.mob_cnt[cell];
@@ -207,13 +243,14 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
         L.soff[e] = fc + mc;
     }
     __syncthreads();
-    const int M = wg_scan(L.soff, ne, L.misc);
+    const int M = wg_scan<NT>(L.soff, ne, L.misc);
+    const unsigned long long tk1 = wall_clock64();
     float acc = 0.0f, cmp = 0.0f;
     int nin = 0, natom = 0;
     uint32_t hs = 0;
-    if (tid == 0) L.misc[16] = 0;
+    if (tid == 0) L.misc[M_NINC] = 0;
     __syncthreads();
-    for (int base = 0; base < M; base += WG) {
+    for (int base = 0; base < M; base += NT) {
         const int m = base + tid;
         bool inc = false;
         int gid = -1;
@@ -225,26 +262,31 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
             const int fc = L.sfix[e];
             float4 p;
             if (local < fc) {
-                p = C.fix_rec[C.fix_start[cell] + local];
+                const int q = C.fix_start[cell] + local;
+                p = C.fix_rec[q];
+                rr = C.fix_rng[q];
                 gid = __float_as_int(p.w);
             } else {
                 const int idx = (cell < C.ncell ? cell * C.cap : C.ovf_base) + (local - fc);
-                gid = C.mob_slot[idx];
-                p = C.res_pos[gid];
+                p = C.mob_rec[idx];
+                gid = __float_as_int(p.w);
+                for (int s = 0; s < C.T; ++s) {
+                    const int4 sp = L.spt[s];
+                    if (gid >= sp.x && gid < sp.x + sp.y) rr = make_int2(sp.z + (gid - sp.x) * sp.w, sp.w);
+                }
             }
             inc = (gid != exclude) && within(cx, cy, cz, p.x, p.y, p.z, C.ax, C.r2max);
             if (inc) {
-                rr = C.res_rng[gid];
                 nin += 1;
                 hs += group_hash((uint32_t)gid);
             }
         }
         const unsigned long long bal = __ballot(inc);
         const int loff = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) L.misc[8 + wave] = __popcll(bal);
+        if (lane == 0) L.misc[M_WCNT + wave] = __popcll(bal);
         __syncthreads();
-        int wbase = L.misc[16];
-        for (int w = 0; w < wave; ++w) wbase += L.misc[8 + w];
+        int wbase = L.misc[M_NINC];
+        for (int w = 0; w < wave; ++w) wbase += L.misc[M_WCNT + w];
         if (inc) {
             const int pos = wbase + loff;
             L.incg[pos] = gid;
@@ -253,15 +295,17 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
         }
         __syncthreads();
         if (tid == 0) {
-            int t = L.misc[16];
-            for (int w = 0; w < NWAVE; ++w) t += L.misc[8 + w];
-            L.misc[16] = t;
+            int t = L.misc[M_NINC];
+            for (int w = 0; w < NW; ++w) t += L.misc[M_WCNT + w];
+            L.misc[M_NINC] = t;
         }
         __syncthreads();
-        const int ninc = L.misc[16];
-        if (ninc > INC_CAP - WG || base + WG >= M) {
-            if (ninc > 0) natom += pair_pass<STAGED>(C, L, ninc, nf, acc, cmp);
-            if (tid == 0) L.misc[16] = 0;
+        const int ninc = L.misc[M_NINC];
+        if (ninc > L.inc_cap - NT || base + NT >= M) {
+            const unsigned long long tp0 = wall_clock64();
+            if (ninc > 0) natom += pair_pass<NT, STAGED>(C, L, ninc, nf, acc, cmp);
+            tpair += wall_clock64() - tp0;
+            if (tid == 0) L.misc[M_NINC] = 0;
             __syncthreads();
         }
     }
@@ -274,23 +318,29 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
     }
     if (lane == 0) {
         L.dred[wave] = v;
-        L.misc[20 + wave] = nin;
-        L.misc[24 + wave] = (int)hs;
+        L.misc[M_NIN + wave] = nin;
+        L.misc[M_HASH + wave] = (int)hs;
     }
     __syncthreads();
     if (tid == 0) {
         double E = 0.0;
         int N = 0;
         uint32_t H = 0;
-        for (int w = 0; w < NWAVE; ++w) {
+        for (int w = 0; w < NW; ++w) {
             E += L.dred[w];
-            N += L.misc[20 + w];
-            H += (uint32_t)L.misc[24 + w];
+            N += L.misc[M_NIN + w];
+            H += (uint32_t)L.misc[M_HASH + w];
         }
         *outE = E;
         *outN = N;
         *outH = H;
         *outA = natom;
+        if (ticks) {
+            const unsigned long long tk2 = wall_clock64();
+            ticks[1] = tk1 - tk0;
+            ticks[2] = (tk2 - tk1) - tpair;
+            ticks[3] = tpair;
+        }
     }
     __syncthreads();
 }
@@ -312,9 +362,6 @@ __device__ inline bool tracing(const Rep &C, int step, int &row)
     return C.tr.k != nullptr && row >= 0 && row < C.tr.max_steps;
 }
 
-// ---- one trial evaluation ----------------------------------------------------
-// item < k: trial item (insert/delete) or E_old (translate/rotate);
-// item >= k: E_new of trial item-k (translate/rotate).
 // Replica data is read and written by workgroups of one XCD (batched form)
 // or by consecutive kernels (plain form).  Counters shared by workgroups use
 // L2 atomics of that XCD (LOCAL) or device-wide atomics.
@@ -325,10 +372,14 @@ __device__ inline void count_add(T *p, T v)
     else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <bool STAGED, bool LOCAL>
+// ---- one trial evaluation ----------------------------------------------------
+// item < k: trial item (insert/delete) or E_old (translate/rotate);
+// item >= k: E_new of trial item-k (translate/rotate).
+template <int NT, bool STAGED, bool LOCAL>
 __device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long tq0 = wall_clock64();
     const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
     const Species &F = C.sp[spi];
     const int nf = F.nf;
@@ -338,7 +389,8 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
         phase = 1;
     }
     if (STAGED)
-        for (int i = tid; i < C.X * C.Y; i += WG) L.pair[i] = C.pair[i];
+        for (int i = tid; i < C.X * C.Y; i += NT) L.pair[i] = C.pair[i];
+    load_species_table(C, L);
     const float4 tcv = C.tc[t];
     const int slot = (kind == K_INS) ? -1 : C.tslot[t];
     const int gid = (kind == K_INS) ? -1 : F.startRes + slot;
@@ -361,8 +413,8 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
             rotate(R, tm.x, tm.y, tm.z, v);
             px = v[0] + cx; py = v[1] + cy; pz = v[2] + cz;
         } else {
-            const int2 rr = C.res_rng[gid];
-            const float4 cur = C.atom[rr.x + a];
+            const int base = F.atomBase + slot * nf;
+            const float4 cur = C.atom[base + a];
             if (kind == K_DEL || phase == 0) {
                 px = cur.x; py = cur.y; pz = cur.z;
             } else if (kind == K_TRN) {
@@ -381,7 +433,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
                 float cen0 = 0.f, cen1 = 0.f, cen2 = 0.f;
                 const float fn = (float)nf;
                 for (int b = 0; b < nf; ++b) {
-                    const float4 q = C.atom[rr.x + b];
+                    const float4 q = C.atom[base + b];
                     cen0 = cen0 + q.x / fn;
                     cen1 = cen1 + q.y / fn;
                     cen2 = cen2 + q.z / fn;
@@ -401,7 +453,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
 
     // adjacency row of trial t over the trial centres (grouping of the accept phase)
     if (kind != K_DEL && phase == 0) {
-        for (int j0 = 0; j0 < k; j0 += WG) {
+        for (int j0 = 0; j0 < k; j0 += NT) {
             const int j = j0 + tid;
             bool b = false;
             if (j < k && j != t) {
@@ -420,10 +472,13 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
     double E;
     int nin, nat;
     uint32_t hh;
-    eval_energy<STAGED>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat);
+    unsigned long long ticks[4];
+    ticks[0] = wall_clock64() - tq0;
+    eval_energy<NT, STAGED>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat, ticks);
     if (tid == 0) {
         C.E[phase * k + t] = E;
         State &W = const_cast<State &>(S);
+        for (int i = 0; i < 4; ++i) count_add<LOCAL>(&W.t_sub[i], ticks[i]);
         count_add<LOCAL>(&W.n_eval, 1ull);
         count_add<LOCAL>(&W.n_pair, (unsigned long long)nat * (unsigned long long)nf);
         count_add<LOCAL>(&W.n_atom, (unsigned long long)nat);
@@ -461,6 +516,7 @@ __device__ inline void cell_insert(const Rep &C, int gid, float x, float y, floa
         C.mob_cnt[C.ncell] = o + 1;
     }
     C.mob_slot[pos] = gid;
+    C.mob_rec[pos] = make_float4(x, y, z, __int_as_float(gid));
     C.mob_loc[gid] = pos;
 }
 
@@ -478,22 +534,22 @@ __device__ inline void cell_remove(const Rep &C, int gid)
     const int n = C.mob_cnt[c] - 1;
     const int moved = C.mob_slot[base + n];
     C.mob_slot[pos] = moved;
+    C.mob_rec[pos] = C.mob_rec[base + n];
     C.mob_loc[moved] = pos;
     C.mob_cnt[c] = n;
     C.mob_loc[gid] = -1;
 }
 
+// group `from` now lives in slot `to` (its centre is unchanged)
 __device__ inline void cell_rename(const Rep &C, int from, int to)
 {
     const int pos = C.mob_loc[from];
     C.mob_slot[pos] = to;
+    float4 r = C.mob_rec[pos];
+    r.w = __int_as_float(to);
+    C.mob_rec[pos] = r;
     C.mob_loc[to] = pos;
     C.mob_loc[from] = -1;
-}
-
-__device__ inline double trial_value(const Rep &C, int kind, int k, int t)
-{
-    return kind == K_INS ? C.E[t] : C.E[k + t] - C.E[t];
 }
 
 // ---- selection and Metropolis (one wave) --------------------------------------
@@ -503,6 +559,7 @@ __device__ inline double trial_value(const Rep &C, int kind, int k, int t)
 //   insert:  min(1, cav/(m P)/(N+1) exp(B - beta E))   (gcmc_move_add.h:164-174)
 //   move:    min(1, 1/(m P) exp(-beta dE))             (gcmc_move_trn.h:215-222)
 // On accept the unused trials within the cutoff of the pick are dropped.
+// Trial values are staged in L.val.
 __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int row, bool trace)
 {
     const int lane = threadIdx.x & 63;
@@ -517,6 +574,8 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
     }
     int N = S.totalNum[spi];
     const int maxN = F.maxNum;
+    const double B = F.B;
+    const double cav = (double)C.cav;
     int g = 0, na = 0;
     double et = S.energy;
     while (true) {
@@ -546,7 +605,7 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
         wave_sync();
         double emin = __builtin_huge_val();
         for (int q = lane; q < mtot; q += 64) {
-            const double v = trial_value(C, kind, k, L.members[q]);
+            const double v = L.val[L.members[q]];
             emin = v < emin ? v : emin;
         }
 #pragma unroll
@@ -554,8 +613,7 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
             const double o2 = __shfl_xor(emin, off, 64);
             emin = o2 < emin ? o2 : emin;
         }
-        for (int q = lane; q < mtot; q += 64)
-            L.w[q] = exp(-BETA * (trial_value(C, kind, k, L.members[q]) - emin));
+        for (int q = lane; q < mtot; q += 64) L.w[q] = exp(-BETA * (L.val[L.members[q]] - emin));
         wave_sync();
         double sumP = 0.0;
         for (int q = 0; q < mtot; ++q) sumP += L.w[q];
@@ -577,10 +635,10 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
             }
         }
         const double P = sumP != 0.0 ? L.w[pq] / sumP : 1.0;
-        const double vp = trial_value(C, kind, k, pick);
+        const double vp = L.val[pick];
         double pp;
         if (ins)
-            pp = ((double)C.cav / ((double)mtot * P)) / ((double)N + 1.0) * exp(F.B - BETA * vp);
+            pp = (cav / ((double)mtot * P)) / ((double)N + 1.0) * exp(B - BETA * vp);
         else
             pp = (1.0 / ((double)mtot * P)) * exp(-BETA * vp);
         const double p = pp < 1.0 ? pp : 1.0;
@@ -612,15 +670,16 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
         wave_sync();
     }
     if (lane == 0) {
-        L.misc[1] = na;
-        L.misc[2] = g;
-        L.misc[3] = N;
+        L.misc[M_ACC] = na;
+        L.misc[M_NG] = g;
+        L.misc[M_N] = N;
         S.energy = et;
         S.acc[spi * 4 + kind] += na;
     }
 }
 
 // ---- accept phase: decisions and commit ------------------------------------------
+template <int NT>
 __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
 {
     const int tid = threadIdx.x;
@@ -629,24 +688,26 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
     const int nf = F.nf;
     int row;
     const bool trace = tracing(C, step, row);
+    // stage the trial values (E for insert/delete, E_new - E_old otherwise)
+    for (int t = tid; t < k; t += NT) L.val[t] = (kind == K_TRN || kind == K_ROT) ? C.E[k + t] - C.E[t] : C.E[t];
     if (tid == 0) {
         S.att[spi * 4 + kind] += 1;
-        L.misc[1] = 0;
-        L.misc[2] = 0;
-        L.misc[3] = S.totalNum[spi];
+        L.misc[M_ACC] = 0;
+        L.misc[M_NG] = 0;
+        L.misc[M_N] = S.totalNum[spi];
     }
     __syncthreads();
     if (kind == K_DEL && k == 1) {
         const int N = S.totalNum[spi];
         if (tid == 0) {
-            const double E = C.E[0];
+            const double E = L.val[0];
             const u32x4 x = draw(C.seed, (uint32_t)step, 0u, P_DECIDE);
             const double ran = u01d(x.z, x.w);
             const double pp = (double)N * exp(-F.B - BETA * (-E));
             const double p = pp < 1.0 ? pp : 1.0;
             const int acc = ran < p;
-            L.misc[1] = acc;
-            L.misc[2] = 1;
+            L.misc[M_ACC] = acc;
+            L.misc[M_NG] = 1;
             if (trace) {
                 C.tr.gpick[row * C.tr.max_k] = 0;
                 C.tr.gacc[row * C.tr.max_k] = acc;
@@ -657,17 +718,18 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
             }
         }
         __syncthreads();
-        if (L.misc[1]) {
+        if (L.misc[M_ACC]) {
             // GupdateDel (gcmc_move_del.h:55-94): swap-remove with the last slot
             const int newN = N - 1;
-            const int hole = F.startRes + C.tslot[0], last = F.startRes + newN;
+            const int hole_slot = C.tslot[0];
+            const int hole = F.startRes + hole_slot, last = F.startRes + newN;
             if (newN > 0 && hole != last) {
-                const int2 rh = C.res_rng[hole], rl = C.res_rng[last];
-                for (int a = tid; a < nf; a += WG) {
-                    const float4 src = C.atom[rl.x + a];
-                    float4 dst = C.atom[rh.x + a];
+                const int bh = F.atomBase + hole_slot * nf, bl = F.atomBase + newN * nf;
+                for (int a = tid; a < nf; a += NT) {
+                    const float4 src = C.atom[bl + a];
+                    float4 dst = C.atom[bh + a];
                     dst.x = src.x; dst.y = src.y; dst.z = src.z;
-                    C.atom[rh.x + a] = dst;
+                    C.atom[bh + a] = dst;
                 }
             }
             if (tid == 0) {
@@ -680,27 +742,28 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
                     cell_rename(C, last, hole);
                 }
                 S.totalNum[spi] = newN;
-                L.misc[3] = newN;
+                L.misc[M_N] = newN;
             }
         }
     } else if (k > 0 && kind != K_DEL) {
         if ((tid >> 6) == 0) greedy_select(C, S, L, row, trace);
         __syncthreads();
-        const int na = L.misc[1];
+        const int na = L.misc[M_ACC];
         // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses
-        for (int idx = tid; idx < na * nf; idx += WG) {
+        for (int idx = tid; idx < na * nf; idx += NT) {
             const int q = idx / nf, a = idx - q * nf;
             const int2 ts = L.accl[q];
-            const int gid = F.startRes + ts.y;
-            const int2 rr = C.res_rng[gid];
+            const int base = F.atomBase + ts.y * nf;
             const float4 p = C.pose[ts.x * MAXF + a];
-            float4 dst = C.atom[rr.x + a];
-            dst.x = p.x; dst.y = p.y; dst.z = p.z;
+            float4 dst;
             if (kind == K_INS) {
-                dst.w = F.tmpl[a].w;
-                C.atype[rr.x + a] = F.ttype[a];
+                dst = make_float4(p.x, p.y, p.z, F.tmpl[a].w);
+                C.atype[base + a] = F.ttype[a];
+            } else {
+                dst = C.atom[base + a];
+                dst.x = p.x; dst.y = p.y; dst.z = p.z;
             }
-            C.atom[rr.x + a] = dst;
+            C.atom[base + a] = dst;
         }
         if (tid == 0) {
             for (int q = 0; q < na; ++q) {
@@ -714,24 +777,18 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
                 } else if (kind == K_TRN) {
                     const float4 old = C.res_pos[gid];
                     C.res_pos[gid] = make_float4(c.x, c.y, c.z, old.w);
-                    const int c_old = cell_index(C.L, C.start, C.inv_h, C.nc, old.x, old.y, old.z);
-                    const int c_new = cell_index(C.L, C.start, C.inv_h, C.nc, c.x, c.y, c.z);
-                    const int pos = C.mob_loc[gid];
-                    const bool in_ovf = pos >= C.ovf_base;
-                    if (c_old != c_new || in_ovf) {
-                        cell_remove(C, gid);
-                        cell_insert(C, gid, c.x, c.y, c.z);
-                    }
+                    cell_remove(C, gid);
+                    cell_insert(C, gid, c.x, c.y, c.z);
                 }
             }
-            if (kind == K_INS) S.totalNum[spi] = L.misc[3];
+            if (kind == K_INS) S.totalNum[spi] = L.misc[M_N];
         }
     }
     if (tid == 0) {
         if (trace) {
             C.tr.k[row] = k;
-            C.tr.ngroups[row] = L.misc[2];
-            C.tr.naccept[row] = L.misc[1];
+            C.tr.ngroups[row] = L.misc[M_NG];
+            C.tr.naccept[row] = L.misc[M_ACC];
             C.tr.n_after[row] = S.totalNum[spi];
             C.tr.etot[row] = S.energy;
         }
@@ -741,6 +798,7 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
 }
 
 // ---- prep phase: picks and trial centres of S.step ------------------------------
+template <int NT>
 __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
 {
     const int tid = threadIdx.x;
@@ -757,12 +815,12 @@ __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
     if (kind != K_INS && k > 0) {
         // partial Fisher-Yates (gcmc_move_trn.h:135-149): pick i takes the value at
         // position J[i] after the swaps 0..i-1, found by tracing the swaps back
-        for (int i = tid; i < k; i += WG) {
+        for (int i = tid; i < k; i += NT) {
             const u32x4 x = draw(C.seed, (uint32_t)step, (uint32_t)i, P_PICK);
             L.J[i] = i + (int)below(x.x, (uint32_t)(N - i));
         }
         __syncthreads();
-        for (int i = tid; i < k; i += WG) {
+        for (int i = tid; i < k; i += NT) {
             int p = L.J[i];
             for (int m = i - 1; m >= 0; --m) {
                 const int jm = L.J[m];
@@ -772,7 +830,7 @@ __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
         }
     }
     __syncthreads();
-    for (int t = tid; t < k; t += WG) {
+    for (int t = tid; t < k; t += NT) {
         float c[3];
         if (kind == K_INS) {
             const u32x4 x = draw(C.seed, (uint32_t)step, (uint32_t)t, P_JITTER);
@@ -814,6 +872,7 @@ __device__ inline void xcd_release() { __builtin_amdgcn_s_waitcnt(0); }
 __device__ inline void xcd_acquire() { asm volatile("buffer_inv sc0" ::: "memory"); }
 
 // advance past empty moves and publish the next move with work (batched form)
+template <int NT>
 __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L)
 {
     while (true) {
@@ -827,7 +886,7 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
             }
             break;
         }
-        prep_step(C, S, L);
+        prep_step<NT>(C, S, L);
         if (S.nitems > 0) {
             if (threadIdx.x == 0) {
                 S.t_pub = wall_clock64();
@@ -840,14 +899,13 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
             break;
         }
-        accept_step(C, S, L);
+        accept_step<NT>(C, S, L);
     }
     __syncthreads();
 }
 
 // ---- kernels ---------------------------------------------------------------------
 
-// start of a run: prepare every replica's current move
 // start of a run: set the end step of every replica and reset the scheduler
 __global__ void k_begin(State *st, const Rep *reps, int r0, int nrep, int nsteps, Sched *sch)
 {
@@ -865,36 +923,36 @@ __global__ void k_begin(State *st, const Rep *reps, int r0, int nrep, int nsteps
 }
 
 // prepare the current move of replicas r0 + blockIdx.x (batched: publish it)
-template <bool STAGED>
-__global__ __launch_bounds__(WG) void k_start(const Rep *reps, State *st, Sched *sch, LdsLayout lay,
+template <int NT, bool STAGED>
+__global__ __launch_bounds__(NT) void k_start(const Rep *reps, State *st, Sched *sch, LdsLayout lay,
                                              int r0, int batched)
 {
     extern __shared__ __align__(16) char smem[];
     const Lds L = carve(smem, lay);
     const int r = r0 + blockIdx.x;
-    if (batched) advance_and_publish(reps[r], st[r], sch, L);
-    else if (st[r].step < st[r].end) prep_step(reps[r], st[r], L);
+    if (batched) advance_and_publish<NT>(reps[r], st[r], sch, L);
+    else if (st[r].step < st[r].end) prep_step<NT>(reps[r], st[r], L);
 }
 
-template <bool STAGED>
-__global__ __launch_bounds__(WG) void k_eval_plain(const Rep *reps, const State *st, int r, LdsLayout lay)
+template <int NT, bool STAGED>
+__global__ __launch_bounds__(NT) void k_eval_plain(const Rep *reps, const State *st, int r, LdsLayout lay)
 {
     extern __shared__ __align__(16) char smem[];
     const State &S = st[r];
     if (S.step >= S.end || (int)blockIdx.x >= S.nitems) return;
     const Lds L = carve(smem, lay);
-    eval_item<STAGED, false>(reps[r], S, blockIdx.x, L);
+    eval_item<NT, STAGED, false>(reps[r], S, blockIdx.x, L);
 }
 
-template <bool STAGED>
-__global__ __launch_bounds__(WG) void k_accept_plain(const Rep *reps, State *st, int r, LdsLayout lay)
+template <int NT, bool STAGED>
+__global__ __launch_bounds__(NT) void k_accept_plain(const Rep *reps, State *st, int r, LdsLayout lay)
 {
     extern __shared__ __align__(16) char smem[];
     State &S = st[r];
     if (S.step >= S.end) return;
     const Lds L = carve(smem, lay);
-    accept_step(reps[r], S, L);
-    if (S.step < S.end) prep_step(reps[r], S, L);
+    accept_step<NT>(reps[r], S, L);
+    if (S.step < S.end) prep_step<NT>(reps[r], S, L);
 }
 
 // Which replicas a workgroup serves: the XCD it runs on (XCC_ID hardware
@@ -914,8 +972,8 @@ __device__ inline int my_xcc() { return (int)(__builtin_amdgcn_s_getreg((20) | (
 // accept and prep phases and publishes the next move.  Exits when every
 // replica finished, or when no replica advanced during spin_limit polls
 // (then sch->abort is set).
-template <bool STAGED>
-__global__ __launch_bounds__(WG) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
+template <int NT, bool STAGED>
+__global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
                                                LdsLayout lay, int spin_limit, XcdMap xm)
 {
     extern __shared__ __align__(16) char smem[];
@@ -928,44 +986,65 @@ __global__ __launch_bounds__(WG) void k_batched(const Rep *reps, State *st, int 
     const int rot = (int)(blockIdx.x / (unsigned)xm.nclass) % nc;
     int idle = 0, seen = -1;
     while (true) {
-        if (tid == 0) {
+        if (tid < 64) {
+            // wave 0 polls every replica of this XCD at once (one L2
+            // read-modify-write per lane) and claims from the first with work
+            const int lane = tid;
             int found = -1, item = 0;
-            const int prog = __hip_atomic_load(&sch->progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int ctl[3] = {0, 0, 0};
+            if (lane == 0) {
+                ctl[0] = __hip_atomic_load(&sch->progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ctl[1] = __hip_atomic_load(&sch->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ctl[2] = __hip_atomic_load(&sch->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const int prog = __shfl(ctl[0], 0, 64), fin = __shfl(ctl[1], 0, 64), ab = __shfl(ctl[2], 0, 64);
             if (prog != seen) {
                 seen = prog;
                 idle = 0;
             }
-            if (__hip_atomic_load(&sch->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= R ||
-                __hip_atomic_load(&sch->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            if (fin >= R || ab != 0) {
                 found = -2;
             } else {
-                for (int j = 0; j < nc; ++j) {
-                    const int r = xm.cls_reps[c0 + (rot + j) % nc];
-                    // an L2 read-modify-write sees the latest claim word of this XCD
-                    const unsigned long long w = __hip_atomic_fetch_or(&st[r].claim, 0ull, __ATOMIC_RELAXED,
-                                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if ((w & 0xFFFFFull) >= ((w >> 20) & 0xFFFFFull)) continue;
-                    const unsigned long long old = __hip_atomic_fetch_add(&st[r].claim, 1ull, __ATOMIC_RELAXED,
-                                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
-                    if (c < n) {
-                        found = r;
-                        item = c;
-                        break;
+                for (int b0 = 0; b0 < nc && found < 0; b0 += 64) {
+                    int r = -1;
+                    unsigned long long w = 0ull;
+                    if (b0 + lane < nc) {
+                        r = xm.cls_reps[c0 + (rot + b0 + lane) % nc];
+                        w = __hip_atomic_fetch_or(&st[r].claim, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    const bool avail = r >= 0 && (w & 0xFFFFFull) < ((w >> 20) & 0xFFFFFull);
+                    unsigned long long bal = __ballot(avail);
+                    while (bal) {
+                        const int l = __ffsll((long long)bal) - 1;
+                        const int rr = __shfl(r, l, 64);
+                        unsigned long long old = 0ull;
+                        if (lane == l)
+                            old = __hip_atomic_fetch_add(&st[rr].claim, 1ull, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                        old = __shfl(old, l, 64);
+                        const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
+                        if (c < n) {
+                            found = rr;
+                            item = c;
+                            break;
+                        }
+                        bal &= bal - 1ull;
                     }
                 }
             }
             if (found == -1 && ++idle > spin_limit) {
                 // no replica advanced during spin_limit polls: stop instead of hanging
-                __hip_atomic_store(&sch->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) __hip_atomic_store(&sch->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 found = -2;
             }
-            L.misc[28] = found;
-            L.misc[29] = item;
+            if (lane == 0) {
+                L.misc[M_FOUND] = found;
+                L.misc[M_ITEM] = item;
+            }
         }
         __syncthreads();
-        const int found = L.misc[28];
-        const int item = L.misc[29];
+        const int found = L.misc[M_FOUND];
+        const int item = L.misc[M_ITEM];
         __syncthreads();
         if (found == -2) break;
         if (found == -1) {
@@ -976,22 +1055,22 @@ __global__ __launch_bounds__(WG) void k_batched(const Rep *reps, State *st, int 
         const Rep &C = reps[found];
         State &S = st[found];
         const unsigned long long te0 = wall_clock64();
-        eval_item<STAGED, true>(C, S, item, L);
+        eval_item<NT, STAGED, true>(C, S, item, L);
         if (tid == 0) count_add<true>(&S.t_eval, wall_clock64() - te0);
         xcd_release();
         __syncthreads();
         if (tid == 0) {
             const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            L.misc[30] = (old == S.nitems - 1) ? 1 : 0;
+            L.misc[M_LAST] = (old == S.nitems - 1) ? 1 : 0;
         }
         __syncthreads();
-        if (L.misc[30]) {
+        if (L.misc[M_LAST]) {
             xcd_acquire();
             const unsigned long long ta0 = wall_clock64();
             if (tid == 0) S.t_step += ta0 - S.t_pub;
-            accept_step(C, S, L);
+            accept_step<NT>(C, S, L);
             const unsigned long long ta1 = wall_clock64();
-            advance_and_publish(C, S, sch, L);
+            advance_and_publish<NT>(C, S, sch, L);
             if (tid == 0) {
                 const unsigned long long ta2 = wall_clock64();
                 S.t_accept += ta1 - ta0;
@@ -1010,8 +1089,8 @@ __global__ void k_xcc_probe(int *out)
 }
 
 // energies of given poses (tests and the adjoint's forward)
-template <bool STAGED>
-__global__ __launch_bounds__(WG) void k_eval_poses(const Rep *reps, int r, int spi, const float *centres,
+template <int NT, bool STAGED>
+__global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int spi, const float *centres,
                                                   const float *poses, const int *excl, double *E, int *ninc,
                                                   uint32_t *hash, LdsLayout lay)
 {
@@ -1021,7 +1100,8 @@ __global__ __launch_bounds__(WG) void k_eval_poses(const Rep *reps, int r, int s
     const Species &F = C.sp[spi];
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x;
     if (STAGED)
-        for (int q = tid; q < C.X * C.Y; q += WG) L.pair[q] = C.pair[q];
+        for (int q = tid; q < C.X * C.Y; q += NT) L.pair[q] = C.pair[q];
+    load_species_table(C, L);
     if (tid < nf) {
         const float *p = poses + ((size_t)i * nf + tid) * 3;
         L.frag[tid] = make_float4(p[0], p[1], p[2], F.kq[tid]);
@@ -1031,7 +1111,8 @@ __global__ __launch_bounds__(WG) void k_eval_poses(const Rep *reps, int r, int s
     double e;
     int n, na;
     uint32_t h;
-    eval_energy<STAGED>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n, &h, &na);
+    eval_energy<NT, STAGED>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n,
+                            &h, &na);
     if (tid == 0) {
         E[i] = e;
         ninc[i] = n;

@@ -18,13 +18,12 @@
 
 namespace gcmc {
 
-constexpr int WG = 256;              // threads of every engine workgroup
-constexpr int NWAVE = WG / 64;
+constexpr int MAXWAVE = 16;          // waves of the largest workgroup (1024 threads)
 constexpr int MAXT = 32;             // species per system
 constexpr int MAXF = 20;             // atoms per fragment (AtomArray.atoms[20])
 constexpr int MAXK = 2048;           // trials per move (confBias)
 constexpr int ADJW = MAXK / 32;      // words of one adjacency row
-constexpr int INC_CAP = 1024;        // included groups buffered before a pair pass
+constexpr int MISC_INTS = 96;        // scratch ints of a workgroup (see engine_kernels.h)
 constexpr double BETA = 1.0 / (0.0083115 * 300.0);  // gcmc/cpp/gcmc.h:40-41
 constexpr double KCOUL = 1388.431112;                // gcmc/cpp/gcmc_energy.h:137
 
@@ -32,7 +31,8 @@ enum Kind { K_INS = 0, K_DEL = 1, K_TRN = 2, K_ROT = 3 };
 
 struct Species {
     int startRes, maxNum, nf, conf;
-    int pad[4];
+    int atomBase;          // first atom of slot 0 (slot j starts at atomBase + j*nf)
+    int pad[3];
     double B;              // beta*muex + ln(L0 L1 L2 conc 6.023e-4)
     float4 tmpl[MAXF];     // centred template (x, y, z, charge)
     float kq[MAXF];        // 1388.431112 * charge, rounded once
@@ -72,7 +72,9 @@ struct Rep {
     const Species *sp;
     const int *fix_start;
     const float4 *fix_rec;   // (x, y, z, gid bits)
+    const int2 *fix_rng;     // (atomStart, atomNum), parallel to fix_rec
     int *mob_cnt, *mob_slot, *mob_loc;
+    float4 *mob_rec;         // bucket entries (x, y, z, gid bits), parallel to mob_slot
     const int *stencil;      // packed (dx+128) | (dy+128)<<8 | (dz+128)<<16
     // per-step buffers
     float4 *tc;              // trial centres
@@ -99,6 +101,9 @@ struct State {
     // and between a move's publication and its last evaluation finishing
     unsigned long long t_eval, t_accept, t_prep, t_step;
     unsigned long long t_pub;
+    // sub-phases of evaluations (ticks, summed): pose+adjacency, stencil
+    // count+scan, candidate tests, pair passes
+    unsigned long long t_sub[4];
 };
 
 struct Sched {
@@ -111,12 +116,14 @@ struct Sched {
 // byte offsets of the dynamic LDS carve-up (host computes, device applies)
 struct LdsLayout {
     int scell, sfix, soff;         // stencil entries (int[nst+2])
-    int incg, incs, incn;          // included groups (int[INC_CAP+1])
+    int incg, incs, incn;          // included groups (int[inc_cap+1])
     int frag, frow;                // fragment pose (float4[MAXF], int[MAXF])
     int pair;                      // float2[X*Y] or -1
-    int misc, dred;                // int[32], double[2*NWAVE]
-    int members, w, accl;          // accept: int[maxk], double[maxk], int2[maxk]
+    int spt;                       // species table int4[MAXT] (startRes, maxNum, atomBase, nf)
+    int misc, dred;                // int[MISC_INTS], double[MAXWAVE]
+    int members, w, val, accl;     // accept: int[maxk], double[maxk] x2, int2[maxk]
     int J;                         // prep: int[maxk]
+    int inc_cap;                   // included groups buffered before a pair pass
     int total;
 };
 

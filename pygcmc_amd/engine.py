@@ -33,7 +33,7 @@ _lib = None
 
 class Tuning(ctypes.Structure):
     _fields_ = [('cell_size', ctypes.c_float), ('persistent_wgs', ctypes.c_int),
-                ('bucket_cap', ctypes.c_int), ('pair_lds', ctypes.c_int)]
+                ('bucket_cap', ctypes.c_int), ('pair_lds', ctypes.c_int), ('wg_size', ctypes.c_int)]
 
 
 class CTrace(ctypes.Structure):
@@ -73,6 +73,7 @@ def lib():
         L.gcmc_engine_last_kernel_ms.argtypes = [P, P]
         L.gcmc_engine_counters.argtypes = [P, I, P]
         L.gcmc_engine_phase_ticks.argtypes = [P, I, P]
+        L.gcmc_engine_sub_ticks.argtypes = [P, I, P]
         L.gcmc_engine_set_trace.argtypes = [P, I, I, I]
         L.gcmc_engine_get_trace.argtypes = [P, I, ctypes.POINTER(CTrace)]
         L.gcmc_engine_eval_poses.argtypes = [P, I, I, I, P, P, P, P, P, P]
@@ -125,9 +126,9 @@ def host_stencil_covers(centres, groups, cryst, startxyz, cutoff, cell_size=0.0)
 class Engine:
     """Independent GCMC systems on one GPU (see module docstring)."""
 
-    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=1):
+    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=1, wg_size=0):
         self._h = ctypes.c_void_p()
-        t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds)
+        t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds, wg_size)
         _check(lib().gcmc_engine_create(ctypes.byref(self._h), ctypes.byref(t)))
         self.systems = []
         self._traces = {}
@@ -195,6 +196,12 @@ class Engine:
         out = np.zeros(4, np.uint64)
         _check(lib().gcmc_engine_phase_ticks(self._h, r, _p(out)))
         return dict(zip(('eval', 'accept', 'prep', 'step'), (float(x) * 1e-8 for x in out)))
+
+    def eval_sub_seconds(self, r=0):
+        """Time inside evaluations by sub-phase, seconds (summed over evaluations)."""
+        out = np.zeros(4, np.uint64)
+        _check(lib().gcmc_engine_sub_ticks(self._h, r, _p(out)))
+        return dict(zip(('pose', 'stencil', 'candidates', 'pairs'), (float(x) * 1e-8 for x in out)))
 
     def step(self, r=0):
         return _check(lib().gcmc_engine_step(self._h, r))

@@ -67,10 +67,10 @@ def test_water_fixture_on_gpu():
         assert abs(E[0] - fx[key]) < 3e-4
 
 
-def _lockstep(sysd, nsteps, mode='plain', max_k=None):
+def _lockstep(sysd, nsteps, mode='plain', max_k=None, wg_size=0):
     max_k = max_k or int(sysd['frag']['confBias'].max())
     gpu_sys = systems.copy(sysd)
-    eng = Engine()
+    eng = Engine(wg_size=wg_size)
     eng.add_system(gpu_sys)
     eng.upload()
     eng.set_trace(0, nsteps, max_k)
@@ -125,19 +125,21 @@ def test_lockstep_water_batched():
     print(f'water batched: max |dE| {err:.3e}, {ng} groups, {mism} mismatches')
 
 
-def test_lockstep_multi_fragment():
+@pytest.mark.parametrize('wg_size', [256, 512, 1024])
+def test_lockstep_multi_fragment(wg_size):
     s = systems.multi(nsteps=400)
-    gt, ot, g, r, st = _lockstep(s, 400, 'batched')
+    gt, ot, g, r, st = _lockstep(s, 400, 'batched', wg_size=wg_size)
     err, ng, mism = _compare(gt, ot, g, r)
     print(f'multi batched: max |dE| {err:.3e}, {ng} groups, {mism} mismatches, N={g["frag"]["totalNum"]}')
 
 
-def test_plain_batched_graph_identical():
+@pytest.mark.parametrize('wg_size', [256, 1024])
+def test_plain_batched_graph_identical(wg_size):
     s = systems.multi(nsteps=300, seed=11)
     outs = []
     for mode in ('plain', 'batched', 'graph'):
         g = systems.copy(s)
-        eng = Engine()
+        eng = Engine(wg_size=wg_size)
         eng.add_system(g)
         eng.upload()
         eng.set_trace(0, 300, int(s['frag']['confBias'].max()))
@@ -158,7 +160,8 @@ def test_plain_batched_graph_identical():
 def test_replicas_match_single_runs():
     """Uneven replicas in one persistent kernel give the same chains as
     each system run alone."""
-    sizes = [(1000, 30.0, 21), (2500, 45.0, 22), (1600, 38.0, 23)]
+    sizes = [(1000, 30.0, 21), (2500, 45.0, 22), (1600, 38.0, 23), (1200, 33.0, 24), (900, 28.0, 25),
+             (2000, 42.0, 26), (1500, 36.0, 27), (1800, 40.0, 28), (1100, 31.0, 29), (2200, 44.0, 30)]
     base = [systems.multi(nsteps=200, n_fixed=n, box=b, seed=sd) for n, b, sd in sizes]
     eng = Engine()
     batch = [systems.copy(s) for s in base]
