@@ -120,6 +120,11 @@ int gcmc_engine_stats(gcmc_engine *e, int replica, double *energy, long long *at
  * mode; 0 otherwise). */
 int gcmc_engine_last_run_ms(gcmc_engine *e, double *ms);
 int gcmc_engine_last_kernel_ms(gcmc_engine *e, double *ms);
+
+/* Persistent-kernel launch shape: out4 = {workgroups requested, workgroups
+ * resident at once (known after the first batched run, else 0), threads per
+ * workgroup, LDS bytes per workgroup}.  No counterpart in the reference. */
+int gcmc_engine_launch_info(gcmc_engine *e, int *out4);
 /* Work done by a replica since upload: [0] trial evaluations, [1] fragment-
  * environment atom pairs, [2] included environment atoms, [3] included
  * groups, [4] fragment atoms, each summed over evaluations. */
@@ -162,6 +167,43 @@ int gcmc_engine_get_trace(gcmc_engine *e, int replica, gcmc_trace *out);
 int gcmc_engine_eval_poses(gcmc_engine *e, int replica, int species, int npose,
                            const float *centres, const float *poses, const int *exclude,
                            double *energy, int *ninc, uint32_t *inchash);
+
+/* ---- adjoint of the pose energy (no counterpart in the reference) ----------
+ * Energy of npose poses of one species (all groups of the replica's current
+ * state whose centre passes the group test around centres[i]; nothing
+ * excluded) with per-type parameters over the ffYNum column types:
+ * eps_kcal[t] (CHARMM epsilon, kcal/mol), rmin_half[t] (CHARMM Rmin/2, A),
+ * dq[t] (charge shift of every atom of type t, e; NULL = 0), combined per
+ * pair by the rule of gcmc/python/simulation.py:63-72.  Writes energy[i]
+ * (kJ/mol), dEdx[i][f][3] = dE_i/dx_f (kJ/mol/A) and partial[i][3][Y] =
+ * dE_i/d(dq, eps_kcal, rmin_half).  All pointers are DEVICE pointers;
+ * poses [npose][nf][3], centres [npose][3].  Work is queued on `stream`
+ * (a hipStream_t; NULL = the default stream) and not waited for. */
+int gcmc_engine_adjoint(gcmc_engine *e, int replica, int species, int npose,
+                        const float *centres, const float *poses, const float *eps_kcal,
+                        const float *rmin_half, const float *dq, double *energy, float *dEdx,
+                        float *partial, void *stream);
+/* Backward of a loss L with dL/dE_i = weights[i] (NULL = 1): grad_pose[i][k]
+ * = weights[i] dEdx[i][k] (per_pose entries per pose) and grad_cols[j] =
+ * sum_i weights[i] partial[i][j] over ncol columns (FP64 sums).  DEVICE
+ * pointers, queued on `stream` (NULL = the default stream).  Not safe to run
+ * concurrently on two streams of one engine (shared column-sum workspace). */
+int gcmc_engine_adjoint_backward(gcmc_engine *e, int npose, int per_pose, int ncol,
+                                 const float *dEdx, const float *partial, const float *weights,
+                                 float *grad_pose, float *grad_cols, void *stream);
+
+/* ---- cavity-bias grid (gcmc/python/dataset.py:21-139) --------------------
+ * Occupancy of the grid_n[0] x grid_n[1] x grid_n[2] cells of edge grid_dx
+ * starting at startxyz by the natom atoms xyz[natom][3] (FP64, host memory):
+ * cell int((x - start)/dx) % n per axis, truncation toward zero and a
+ * non-negative remainder as the reference's Python computes them.  Writes
+ * the free-cell count to *npoints and, when grid is not NULL and the count is
+ * at most max_points, the corners (start + i dx, FP64 rounded to FP32) of the
+ * free cells to grid[3*npoints] in linear order (x slowest, z fastest).  The
+ * reference lists the same cells in CPython set order.  Runs on the current
+ * HIP device; synchronous. */
+int gcmc_cavity_grid(int natom, const double *xyz, const double *startxyz, double grid_dx,
+                     const int *grid_n, int max_points, float *grid, int *npoints);
 
 /* Host execution of the engine's FP32 group-inclusion test -- the code the
  * GPU runs, compiled for the CPU -- for n pairs of points: out[i] = 1 when

@@ -22,6 +22,9 @@
  *                           gcmc_move_trn.h:125-253, gcmc_move_rot.h:151-275
  *   update semantics ...... GupdateAdd/Del/Trn/Rot (gcmc_move_*.h)
  *   driver loop ........... gcmc/cpp/gcmc.cu:77-98 (move = type*4 + kind)
+ *   pair-table rule ....... gcmc/python/simulation.py:63-72 (sigma mean,
+ *                           epsilon geometric mean) -- used by orc_adjoint,
+ *                           whose derivatives have no reference counterpart
  *
  * Declared deviations, identical in the product (both follow one definition):
  *   - variates come from Philox4x32-10 (Salmon et al., SC'11) keyed by
@@ -676,3 +679,106 @@ int orc_run_c(const orc_info *info, orc_frag *frag, orc_res *res, orc_atom *atom
 
 uint32_t orc_group_hash_term(uint32_t g) { return orc_mix(g); }
 uint32_t orc_pose_hash(int nf, const float *p) { return orc_posehash(nf, (const float (*)[3])p); }
+
+/* ---- adjoint of a pose energy (no reference counterpart) ------------------
+ * E(pose; eps, rmin, dq) with per-type parameters: type t has CHARMM
+ * epsilon eps[t] (kcal/mol) and Rmin/2 rmin[t] (A), i.e. sigma_t =
+ * rmin[t] 2^(5/6)/10 nm and epsilon_t = 4.184 eps[t] kJ/mol, combined as in
+ * gcmc/python/simulation.py:63-72; every atom of type t carries its own
+ * charge plus dq[t].  Groups are included by the FP32 group-centre test of
+ * calcEnergy.  Outputs (any may be NULL): E, dE/dx of the fragment atoms
+ * [nf][3], and dE/d dq[t], dE/d eps[t], dE/d rmin[t] over Y types, all FP64.
+ */
+static void orc_adj_group(const orc_sys *S, int g, const float c[3], int nf,
+                          const float (*pose)[3], const orc_atom *tmpl,
+                          const double *sig, const double *epk, const double *eps,
+                          const double *dq, double *E, double *gx, double *gq,
+                          double *ge, double *gr)
+{
+    const orc_res *r = &S->res[g];
+    if (!orc_within(c, r->position, S->info->cryst, S->info->cutoff)) return;
+    const float *L = S->info->cryst;
+    const double cs = pow(2.0, 5.0 / 6.0) / 10.0;
+    for (int a = r->atomStart; a < r->atomStart + r->atomNum; ++a) {
+        const orc_atom *A = &S->atom[a];
+        const int ta = A->type;
+        const double qa = (double)A->charge + dq[ta];
+        for (int f = 0; f < nf; ++f) {
+            const int tf = tmpl[f].type;
+            const double qf = (double)tmpl[f].charge + dq[tf];
+            double d[3], r2 = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                double Li = (double)L[i];
+                double di = (double)A->position[i] - (double)pose[f][i];
+                double q = di / Li;
+                double rr = q >= 0.0 ? floor(q + 0.5) : ceil(q - 0.5);
+                d[i] = di - rr * Li;
+                r2 += d[i] * d[i];
+            }
+            const double sg = 0.5 * (sig[tf] + sig[ta]);
+            const double ep = sqrt(eps[tf] * eps[ta]);
+            const double s2 = sg * sg * 100.0 / r2;
+            const double s6 = s2 * s2 * s2, s12 = s6 * s6;
+            const double lj = 4.0 * ep * (s12 - s6);
+            const double ir = 1.0 / sqrt(r2);
+            const double el = ORC_KCOUL * qf * qa * ir;
+            *E += lj + el;
+            if (gx) {
+                /* dE/d(r^2); d = x_a - x_f so dE/dx_f = -2 dE/dr2 d */
+                const double dr2 = 4.0 * ep * (-6.0 * s12 + 3.0 * s6) / r2 - 0.5 * el / r2;
+                for (int i = 0; i < 3; ++i) gx[3 * f + i] += -2.0 * dr2 * d[i];
+            }
+            if (gq) {
+                gq[tf] += ORC_KCOUL * qa * ir;
+                gq[ta] += ORC_KCOUL * qf * ir;
+            }
+            if (ge) {
+                /* d eps_pair / d eps[t] = eps_pair / (2 eps[t]) per matching side */
+                if (epk[tf] > 0.0) ge[tf] += lj / (2.0 * epk[tf]);
+                if (epk[ta] > 0.0) ge[ta] += lj / (2.0 * epk[ta]);
+            }
+            if (gr) {
+                /* dE/dsigma_pair * dsigma_pair/drmin[t] = ... * cs/2 per side */
+                const double ds = 4.0 * ep * (12.0 * s12 - 6.0 * s6) / sg * 0.5 * cs;
+                gr[tf] += ds;
+                gr[ta] += ds;
+            }
+        }
+    }
+}
+
+double orc_adjoint(const orc_info *info, const orc_frag *frag, const orc_res *res,
+                   const orc_atom *atom, int species, const float *centre, const float *pose,
+                   int exclude, const double *eps_kcal, const double *rmin_half,
+                   const double *dq, double *gx, double *gq, double *ge, double *gr)
+{
+    orc_sys S = {info, (orc_frag *)frag, (orc_res *)res, (orc_atom *)atom, NULL, NULL, NULL};
+    const int Y = info->ffYNum;
+    double *sig = (double *)malloc(sizeof(double) * 3 * (size_t)Y);
+    double *eps = sig + Y, *epk = sig + 2 * Y;
+    const double cs = pow(2.0, 5.0 / 6.0) / 10.0;
+    for (int t = 0; t < Y; ++t) {
+        sig[t] = rmin_half[t] * cs;
+        epk[t] = fabs(eps_kcal[t]);
+        eps[t] = 4.184 * epk[t];
+    }
+    const orc_frag *F = &frag[species];
+    const int nf = F->num_atoms;
+    if (gx) memset(gx, 0, sizeof(double) * 3 * (size_t)nf);
+    if (gq) memset(gq, 0, sizeof(double) * (size_t)Y);
+    if (ge) memset(ge, 0, sizeof(double) * (size_t)Y);
+    if (gr) memset(gr, 0, sizeof(double) * (size_t)Y);
+    double E = 0.0;
+    const float (*P)[3] = (const float (*)[3])pose;
+    const int nfix = frag[0].startRes;
+    for (int g = 0; g < nfix; ++g)
+        orc_adj_group(&S, g, centre, nf, P, F->atoms, sig, epk, eps, dq, &E, gx, gq, ge, gr);
+    for (int s = 0; s < info->fragTypeNum; ++s) {
+        const int g0 = frag[s].startRes, g1 = g0 + frag[s].totalNum;
+        for (int g = g0; g < g1; ++g)
+            if (g != exclude)
+                orc_adj_group(&S, g, centre, nf, P, F->atoms, sig, epk, eps, dq, &E, gx, gq, ge, gr);
+    }
+    free(sig);
+    return E;
+}

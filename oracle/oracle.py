@@ -54,6 +54,8 @@ def lib():
         L.orc_group_hash_term.restype = ctypes.c_uint32
         L.orc_pose_hash.argtypes = [ctypes.c_int, P]
         L.orc_pose_hash.restype = ctypes.c_uint32
+        L.orc_adjoint.argtypes = [P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P, P, P, P, P, P, P]
+        L.orc_adjoint.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -97,6 +99,36 @@ def picks(seed, step, N, n):
 
 def species_B(sysd, s):
     return lib().orc_species_B(_p(sysd['info']), _p(sysd['frag'][s:s + 1]))
+
+
+def type_params(sysd):
+    """Per-type CHARMM epsilon (kcal/mol) and Rmin/2 (A) over the all-types
+    list of a synthetic system (the parameters its pair table was built from)."""
+    from pygcmc_amd.dataset import gromacs_to_charmm
+    eps, rmin = [], []
+    for t in sysd['types2']:
+        r, e = gromacs_to_charmm(*sysd['nb'][t])
+        eps.append(e)
+        rmin.append(r)
+    return np.array(eps, np.float64), np.array(rmin, np.float64)
+
+
+def adjoint(sysd, species, centre, pose, eps, rmin, dq=None, exclude=-1):
+    """FP64 energy and gradients of one pose with per-type parameters:
+    returns (E, dE/dx [nf,3], dE/ddq [Y], dE/deps [Y], dE/drmin [Y])."""
+    Y = int(sysd['info'][0]['ffYNum'])
+    nf = int(sysd['frag'][species]['num_atoms'])
+    c = np.ascontiguousarray(centre, np.float32)
+    p = np.ascontiguousarray(pose, np.float32)
+    eps = np.ascontiguousarray(eps, np.float64)
+    rmin = np.ascontiguousarray(rmin, np.float64)
+    dq = np.zeros(Y) if dq is None else np.ascontiguousarray(dq, np.float64)
+    gx = np.zeros((nf, 3))
+    gq, ge, gr = np.zeros(Y), np.zeros(Y), np.zeros(Y)
+    e = lib().orc_adjoint(_p(sysd['info']), _p(sysd['frag']), _p(sysd['res']), _p(sysd['atoms']),
+                          species, _p(c), _p(p), exclude, _p(eps), _p(rmin), _p(dq),
+                          _p(gx), _p(gq), _p(ge), _p(gr))
+    return e, gx, gq, ge, gr
 
 
 def energy_pose(sysd, species, centre, pose, exclude=-1):

@@ -24,7 +24,7 @@ MODULE = os.path.join(HERE, 'gpu' + EXT_SUFFIX)
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('PYGCMC_ARCH', 'gfx950')
 
-HIP_SOURCES = ['engine.hip']
+HIP_SOURCES = ['engine.hip', 'grid.hip']
 
 HIP_FLAGS = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',

@@ -16,7 +16,7 @@
 #include <vector>
 
 #include "../../include/pygcmc_hip.h"
-#include "engine_kernels.h"
+#include "adjoint_kernels.h"
 
 static_assert(sizeof(Atom) == 20, "Atom ABI");
 static_assert(sizeof(AtomArray) == 436, "AtomArray ABI");
@@ -126,7 +126,9 @@ struct gcmc_engine {
     Sched *d_sched = nullptr;
     LdsLayout lay{};
     bool staged = false;
-    int nwg = 0;
+    int nwg = 0;               // persistent workgroups requested
+    int ncu = 0;
+    int resident = 0;          // persistent workgroups that fit on the device at once
     int maxk = 1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evk0 = nullptr, evk1 = nullptr;
@@ -140,6 +142,8 @@ struct gcmc_engine {
     int nt = 256;              // threads per workgroup
     XcdMap xm{};
     int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]
+    double *d_adj_work = nullptr;  // column-sum partials of the adjoint backward
+    size_t adj_work_n = 0;
 };
 
 namespace {
@@ -470,8 +474,17 @@ void launch_batched(gcmc_engine *E, int nsteps)
                        nsteps, E->d_sched);
     hipLaunchKernelGGL((k_start<NT, S>), dim3(R), dim3(NT), lds, E->stream, E->d_rep, E->d_state, E->d_sched,
                        E->lay, 0, 1);
+    if (E->resident <= 0) {
+        // workgroups beyond the resident ones would only start once the
+        // others have exited: launch no more than fit
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_batched<NT, S>, NT, lds) != hipSuccess || occ < 1)
+            occ = 1;
+        E->resident = occ * std::max(E->ncu, 1);
+    }
+    const int nwg = std::min(E->nwg, E->resident);
     if (E->timed_kernel) (void)hipEventRecord(E->evk0, E->stream);
-    hipLaunchKernelGGL((k_batched<NT, S>), dim3(E->nwg), dim3(NT), lds, E->stream, E->d_rep, E->d_state, R,
+    hipLaunchKernelGGL((k_batched<NT, S>), dim3(nwg), dim3(NT), lds, E->stream, E->d_rep, E->d_state, R,
                        E->d_sched, E->lay, E->spin_limit, E->xm);
     if (E->timed_kernel) (void)hipEventRecord(E->evk1, E->stream);
 }
@@ -578,6 +591,11 @@ int sync_rep_struct(gcmc_engine *E, int r)
 
 }  // namespace
 
+namespace gcmc_internal {
+// shared by the library's other translation units (one error channel)
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace gcmc_internal
+
 extern "C" {
 
 const char *gcmc_last_error(void) { return g_err.c_str(); }
@@ -619,6 +637,7 @@ int gcmc_engine_destroy(gcmc_engine *E)
     if (E->d_state) hipFree(E->d_state);
     if (E->d_sched) hipFree(E->d_sched);
     if (E->d_cls) hipFree(E->d_cls);
+    if (E->d_adj_work) hipFree(E->d_adj_work);
     if (E->ev0) hipEventDestroy(E->ev0);
     if (E->ev1) hipEventDestroy(E->ev1);
     if (E->evk0) hipEventDestroy(E->evk0);
@@ -689,7 +708,9 @@ int gcmc_engine_upload(gcmc_engine *E)
     int dev = 0, ncu = 0;
     HIPX(hipGetDevice(&dev));
     HIPX(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    E->nwg = E->tune.persistent_wgs > 0 ? E->tune.persistent_wgs : (1024 / E->nt) * std::max(ncu, 1);
+    E->ncu = ncu;
+    E->resident = 0;
+    E->nwg = E->tune.persistent_wgs > 0 ? E->tune.persistent_wgs : (2048 / E->nt) * std::max(ncu, 1);
     rc = build_xcd_map(E, ncu);
     if (rc) return rc;
     E->uploaded = true;
@@ -770,6 +791,17 @@ int gcmc_engine_last_run_ms(gcmc_engine *E, double *ms)
     if (!E || !ms) return fail(GCMC_E_ARG, "null argument");
     *ms = E->last_ms;
     return 0;
+}
+
+int gcmc_engine_launch_info(gcmc_engine *E, int *out4)
+{
+    if (!E || !out4) return fail(GCMC_E_ARG, "null argument");
+    if (!E->uploaded) return fail(GCMC_E_STATE, "upload() first");
+    out4[0] = E->nwg;
+    out4[1] = E->resident;
+    out4[2] = E->nt;
+    out4[3] = E->lay.total;
+    return GCMC_OK;
 }
 
 int gcmc_engine_last_kernel_ms(gcmc_engine *E, double *ms)
@@ -973,6 +1005,71 @@ int gcmc_engine_eval_poses(gcmc_engine *E, int r, int species, int npose, const 
     if (e == hipSuccess && inchash) e = hipMemcpy(inchash, dh, npose * sizeof(uint32_t), hipMemcpyDeviceToHost);
     B.release();
     if (e != hipSuccess) return fail(GCMC_E_HIP, std::string("eval_poses: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int gcmc_engine_adjoint(gcmc_engine *E, int r, int species, int npose, const float *centres,
+                        const float *poses, const float *eps_kcal, const float *rmin_half, const float *dq,
+                        double *energy, float *dEdx, float *partial, void *stream)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HostRep &H = *E->reps[r];
+    if (species < 0 || species >= H.info.fragTypeNum) return fail(GCMC_E_ARG, "bad species");
+    if (npose < 0) return fail(GCMC_E_ARG, "negative pose count");
+    if (npose == 0) return 0;
+    if (!centres || !poses || !eps_kcal || !rmin_half || !energy || !dEdx || !partial)
+        return fail(GCMC_E_ARG, "null array");
+    const int nf = H.frag[species].num_atoms;
+    const int X = H.rep.X, Y = H.rep.Y;
+    const int adj_off = (E->lay.total + 15) & ~15;
+    const int lds = adj_off + adj_lds_bytes(X, Y);
+    if (lds > 65536) return fail(GCMC_E_ARG, "adjoint LDS layout exceeds 64 KiB (too many atom types)");
+    hipStream_t s = (hipStream_t)stream;
+#define ADJ_LAUNCH(NFV)                                                                                         \
+    hipLaunchKernelGGL((k_adjoint<256, NFV>), dim3(npose), dim3(256), lds, s, E->d_rep, r, species, centres, poses, \
+                       eps_kcal, rmin_half, dq, energy, dEdx, partial, E->lay, adj_off)
+    if (nf <= 4) ADJ_LAUNCH(4);
+    else if (nf <= 8) ADJ_LAUNCH(8);
+    else if (nf <= 12) ADJ_LAUNCH(12);
+    else if (nf <= 16) ADJ_LAUNCH(16);
+    else ADJ_LAUNCH(MAXF);
+#undef ADJ_LAUNCH
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GCMC_E_HIP, std::string("adjoint: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int gcmc_engine_adjoint_backward(gcmc_engine *E, int npose, int per_pose, int ncol, const float *dEdx,
+                                 const float *partial, const float *weights, float *grad_pose, float *grad_cols,
+                                 void *stream)
+{
+    if (!E) return fail(GCMC_E_ARG, "null engine");
+    if (npose < 0 || per_pose < 0 || ncol < 0) return fail(GCMC_E_ARG, "negative size");
+    if ((npose > 0 && per_pose > 0 && (!dEdx || !grad_pose)) || (ncol > 0 && (!partial || !grad_cols)))
+        return fail(GCMC_E_ARG, "null array");
+    hipStream_t s = (hipStream_t)stream;
+    constexpr int NCH = 32;
+    const size_t need = (size_t)NCH * std::max(ncol, 1);
+    if (E->adj_work_n < need) {
+        if (E->d_adj_work) (void)hipFree(E->d_adj_work);
+        E->d_adj_work = nullptr;
+        E->adj_work_n = 0;
+        HIPX(hipMalloc(&E->d_adj_work, need * sizeof(double)));
+        E->adj_work_n = need;
+    }
+    const size_t nx = (size_t)npose * per_pose;
+    if (nx > 0)
+        hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, npose, per_pose, dEdx,
+                           weights, grad_pose);
+    if (ncol > 0) {
+        hipLaunchKernelGGL(k_adjoint_colsum, dim3((ncol + 63) / 64, NCH), dim3(1024), 0, s, npose, ncol, partial,
+                           weights, E->d_adj_work);
+        hipLaunchKernelGGL(k_adjoint_chunksum, dim3((ncol + 255) / 256), dim3(256), 0, s, NCH, ncol, E->d_adj_work,
+                           grad_cols);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GCMC_E_HIP, std::string("adjoint backward: ") + hipGetErrorString(e));
     return 0;
 }
 

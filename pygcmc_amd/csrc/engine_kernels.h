@@ -192,6 +192,15 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
     return A;
 }
 
+// the energy pass of eval_energy
+struct EnergyPass {
+    template <int NT, bool STAGED>
+    __device__ int run(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
+    {
+        return pair_pass<NT, STAGED>(C, L, n, nf, acc, cmp);
+    }
+};
+
 // species table in LDS (mobile group id -> atom range)
 __device__ inline void load_species_table(const Rep &C, const Lds &L)
 {
@@ -206,10 +215,11 @@ __device__ inline void load_species_table(const Rep &C, const Lds &L)
 // centre plus the overflow bucket; kept when gid != exclude and the FP32
 // distanceP test passes (gcmc_energy.h:152-191).  Needs L.spt loaded.
 // Thread 0 receives the results.
-template <int NT, bool STAGED>
-__device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, int nf, int exclude,
-                                   const Lds &L, double *outE, int *outN, uint32_t *outH, int *outA,
-                                   unsigned long long *ticks = nullptr)
+template <int NT, bool STAGED, class Pass>
+__device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep &C, float cx, float cy, float cz,
+                                                                       int nf, int exclude, const Lds &L, Pass &P,
+                                                                       double *outE, int *outN, uint32_t *outH,
+                                                                       int *outA, unsigned long long *ticks)
 {
     unsigned long long tk0 = wall_clock64(), tpair = 0;
     constexpr int NW = NT / 64;
@@ -303,7 +313,7 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
         const int ninc = L.misc[M_NINC];
         if (ninc > L.inc_cap - NT || base + NT >= M) {
             const unsigned long long tp0 = wall_clock64();
-            if (ninc > 0) natom += pair_pass<NT, STAGED>(C, L, ninc, nf, acc, cmp);
+            if (ninc > 0) natom += P.template run<NT, STAGED>(C, L, ninc, nf, acc, cmp);
             tpair += wall_clock64() - tp0;
             if (tid == 0) L.misc[M_NINC] = 0;
             __syncthreads();
@@ -343,6 +353,16 @@ __device__ inline void eval_energy(const Rep &C, float cx, float cy, float cz, i
         }
     }
     __syncthreads();
+}
+
+template <int NT, bool STAGED>
+__device__ __attribute__((always_inline)) inline void eval_energy(const Rep &C, float cx, float cy, float cz, int nf,
+                                                                  int exclude, const Lds &L, double *outE, int *outN,
+                                                                  uint32_t *outH, int *outA,
+                                                                  unsigned long long *ticks = nullptr)
+{
+    EnergyPass P;
+    eval_energy_pass<NT, STAGED>(C, cx, cy, cz, nf, exclude, L, P, outE, outN, outH, outA, ticks);
 }
 
 __device__ inline uint32_t pose_hash(const float4 *p, int nf)
@@ -987,8 +1007,10 @@ __global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int 
     int idle = 0, seen = -1;
     while (true) {
         if (tid < 64) {
-            // wave 0 polls every replica of this XCD at once (one L2
-            // read-modify-write per lane) and claims from the first with work
+            // wave 0 polls every replica of this XCD at once, one claim word
+            // per lane.  The polls are loads that bypass L1 (sc0) and are served
+            // by the XCD's L2 in parallel; read-modify-writes of one address
+            // serialize in L2, so an atomic is issued only to claim.
             const int lane = tid;
             int found = -1, item = 0;
             int ctl[3] = {0, 0, 0};
@@ -1010,12 +1032,20 @@ __global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int 
                     unsigned long long w = 0ull;
                     if (b0 + lane < nc) {
                         r = xm.cls_reps[c0 + (rot + b0 + lane) % nc];
-                        w = __hip_atomic_fetch_or(&st[r].claim, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        w = __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                     }
-                    const bool avail = r >= 0 && (w & 0xFFFFFull) < ((w >> 20) & 0xFFFFFull);
-                    unsigned long long bal = __ballot(avail);
-                    while (bal) {
-                        const int l = __ffsll((long long)bal) - 1;
+                    bool avail = r >= 0 && (w & 0xFFFFFull) < ((w >> 20) & 0xFFFFFull);
+                    // the replica furthest behind (lowest step) first: the run
+                    // lasts as long as its slowest chain
+                    const unsigned step = (unsigned)(w >> 40);
+                    while (__ballot(avail) != 0ull) {
+                        unsigned key = avail ? ((step << 6) | (unsigned)lane) : 0xFFFFFFFFu;
+#pragma unroll
+                        for (int off = 32; off > 0; off >>= 1) {
+                            const unsigned o2 = __shfl_xor(key, off, 64);
+                            key = o2 < key ? o2 : key;
+                        }
+                        const int l = (int)(key & 63u);
                         const int rr = __shfl(r, l, 64);
                         unsigned long long old = 0ull;
                         if (lane == l)
@@ -1028,7 +1058,7 @@ __global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int 
                             item = c;
                             break;
                         }
-                        bal &= bal - 1ull;
+                        if (lane == l) avail = false;
                     }
                 }
             }

@@ -26,6 +26,7 @@ constexpr int ADJW = MAXK / 32;      // words of one adjacency row
 constexpr int MISC_INTS = 96;        // scratch ints of a workgroup (see engine_kernels.h)
 constexpr double BETA = 1.0 / (0.0083115 * 300.0);  // gcmc/cpp/gcmc.h:40-41
 constexpr double KCOUL = 1388.431112;                // gcmc/cpp/gcmc_energy.h:137
+constexpr float KCOUL_F = 1388.431112f;
 
 enum Kind { K_INS = 0, K_DEL = 1, K_TRN = 2, K_ROT = 3 };
 

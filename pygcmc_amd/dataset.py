@@ -155,6 +155,11 @@ def charmm_to_gromacs(eps_kcal, rmin_half):
     return rmin_half * 2.0 ** (5.0 / 6.0) / 10.0, 4.184 * abs(eps_kcal)
 
 
+def gromacs_to_charmm(sigma_nm, eps_kj):
+    """Inverse of ``charmm_to_gromacs``: (Rmin/2 in A, epsilon in kcal/mol)."""
+    return sigma_nm * 10.0 / 2.0 ** (5.0 / 6.0), eps_kj / 4.184
+
+
 def max_num(conc, volume, n_insert, n_initial):
     """``maxNum = max(conc*V*6.023e-4*2 + n_insert, N0 + n_insert)`` truncated to
     int (``gcmc/python/dataset.py:210-213``)."""

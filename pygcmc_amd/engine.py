@@ -71,6 +71,7 @@ def lib():
         L.gcmc_engine_stats.argtypes = [P, I, P, P, P]
         L.gcmc_engine_last_run_ms.argtypes = [P, P]
         L.gcmc_engine_last_kernel_ms.argtypes = [P, P]
+        L.gcmc_engine_launch_info.argtypes = [P, P]
         L.gcmc_engine_counters.argtypes = [P, I, P]
         L.gcmc_engine_phase_ticks.argtypes = [P, I, P]
         L.gcmc_engine_sub_ticks.argtypes = [P, I, P]
@@ -185,6 +186,13 @@ class Engine:
         _check(lib().gcmc_engine_last_kernel_ms(self._h, ctypes.byref(ms)))
         return ms.value
 
+    def launch_info(self):
+        """Persistent-kernel shape: requested and resident workgroups, threads
+        per workgroup, LDS bytes per workgroup."""
+        out = np.zeros(4, np.int32)
+        _check(lib().gcmc_engine_launch_info(self._h, _p(out)))
+        return dict(zip(('wgs', 'resident', 'threads', 'lds_bytes'), (int(x) for x in out)))
+
     def counters(self, r=0):
         """Work since upload: evals, pairs, env atoms, groups, fragment atoms."""
         out = np.zeros(5, np.uint64)
@@ -250,3 +258,33 @@ class Engine:
 def run_gcmc(info, frag, res, atoms, grid, ff, moves):
     """ctypes path to ``runGCMC_hip`` (same contract as ``pygcmc_amd.gpu.runGCMC``)."""
     _check(lib().runGCMC_hip(_p(info), _p(frag), _p(res), _p(atoms), _p(grid), _p(ff), _p(moves)))
+
+
+def cavity_grid(xyz, cryst, grid_dx=1.0, cavity_bias=True):
+    """Cavity-bias grid built on the GPU (``gcmc_cavity_grid``): same
+    definition and result as ``dataset.get_grid(order='linear')``
+    (``gcmc/python/dataset.py:21-139``).  Returns ``(startxyz, grid_n,
+    grid float32[3G], cavity_factor)``."""
+    from .dataset import box_start
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    start, end = box_start(xyz, cryst)
+    if not cavity_bias:
+        grid_dx = 1.0
+    n = [int((end[i] - start[i]) / grid_dx) for i in range(3)]
+    L = lib()
+    if not getattr(L, '_grid_types_set', False):
+        L.gcmc_cavity_grid.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L._grid_types_set = True
+    st = np.array(start, np.float64)
+    nn = np.array(n, np.int32)
+    cnt = ctypes.c_int()
+    marks = xyz if cavity_bias else xyz[:0]
+    _check(L.gcmc_cavity_grid(len(marks), _p(marks), _p(st), float(grid_dx), _p(nn), 0, None,
+                              ctypes.byref(cnt)))
+    grid = np.empty(3 * cnt.value, np.float32)
+    _check(L.gcmc_cavity_grid(len(marks), _p(marks), _p(st), float(grid_dx), _p(nn), cnt.value,
+                              _p(grid), ctypes.byref(cnt)))
+    total = n[0] * n[1] * n[2]
+    cav = cnt.value / total if cavity_bias else 1.0
+    return start, n, grid, cav

@@ -90,3 +90,47 @@ def water3_system(charges=True, lj=True, shift=(30.0, 40.0, 150.0)):
     pose = W[2].astype(np.float32)
     centre = pose.mean(axis=0).astype(np.float32)
     return sysd, centre, pose, fx
+
+
+def moderate_poses(sysd, species, n, seed=0, emax=200.0):
+    """n random poses (random_poses) whose FP64 energy lies in [-emax, emax]:
+    no overlapping atoms, so gradients are well conditioned."""
+    cs, ps = [], []
+    s = seed
+    while len(cs) < n:
+        c, p = random_poses(sysd, species, 4 * n, seed=s)
+        s += 1
+        for i in range(len(c)):
+            if abs(O.energy_pose(sysd, species, c[i], p[i])[0]) <= emax:
+                cs.append(c[i])
+                ps.append(p[i])
+                if len(cs) == n:
+                    break
+    return np.array(cs, np.float32), np.array(ps, np.float32)
+
+
+def fd_pose_gradient(sysd, species, centre, pose, eps, rmin, h=1e-4):
+    """Central differences of the FP64 oracle energy in each pose coordinate.
+    The pose is FP32, so the step is the one actually represented."""
+    g = np.zeros(pose.shape, np.float64)
+    for f in range(pose.shape[0]):
+        for k in range(3):
+            pp, pm = pose.copy(), pose.copy()
+            pp[f, k] = np.float32(pose[f, k] + h)
+            pm[f, k] = np.float32(pose[f, k] - h)
+            ep = O.adjoint(sysd, species, centre, pp, eps, rmin)[0]
+            em = O.adjoint(sysd, species, centre, pm, eps, rmin)[0]
+            g[f, k] = (ep - em) / (float(pp[f, k]) - float(pm[f, k]))
+    return g
+
+
+def fd_type_gradient(sysd, species, centre, pose, eps, rmin, which, t, h=1e-4):
+    """Central difference of the FP64 oracle energy in one per-type parameter
+    (which: 'q' charge shift, 'eps', 'rmin')."""
+    Y = len(eps)
+    out = []
+    for sgn in (1.0, -1.0):
+        e, r, dq = eps.copy(), rmin.copy(), np.zeros(Y)
+        {'q': dq, 'eps': e, 'rmin': r}[which][t] += sgn * h
+        out.append(O.adjoint(sysd, species, centre, pose, e, r, dq)[0])
+    return (out[0] - out[1]) / (2 * h)

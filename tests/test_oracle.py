@@ -265,3 +265,52 @@ def test_brute_force_small_system():
                     tot += 4 * ep * (s6 * s6 - s6) + COULOMB * float(frag['atoms'][f]['charge']) * \
                         float(A['charge']) / math.sqrt(r2)
         assert O.energy_pose(s, 0, c[i], p[i])[0] == pytest.approx(tot, rel=1e-12, abs=1e-9)
+
+
+# ---- adjoint of the pose energy (FP64 oracle) ---------------------------------
+
+
+def _adjoint_case():
+    s = systems.multi(nsteps=10, n_fixed=3000, box=45.0)
+    eps, rmin = O.type_params(s)
+    return s, eps, rmin
+
+
+def test_adjoint_energy_equals_table_energy():
+    s, eps, rmin = _adjoint_case()
+    c, p = systems.random_poses(s, 0, 20, seed=5)
+    for i in range(len(c)):
+        e_tab = O.energy_pose(s, 0, c[i], p[i])[0]
+        e_par = O.adjoint(s, 0, c[i], p[i], eps, rmin)[0]
+        # the pair table holds float32 (sigma, eps); the adjoint recombines
+        assert abs(e_par - e_tab) <= 1e-6 * max(1.0, abs(e_tab))
+
+
+def test_adjoint_gradients_match_finite_differences():
+    s, eps, rmin = _adjoint_case()
+    for spi in (0, 5, 8):
+        c, p = systems.moderate_poses(s, spi, 2, seed=11 + spi)
+        for i in range(len(c)):
+            e, gx, gq, ge, gr = O.adjoint(s, spi, c[i], p[i], eps, rmin)
+            fd = systems.fd_pose_gradient(s, spi, c[i], p[i], eps, rmin)
+            assert np.linalg.norm(gx - fd) <= 1e-6 * np.linalg.norm(fd) + 1e-6
+            used = np.nonzero(np.abs(gq) + np.abs(ge) + np.abs(gr))[0]
+            assert len(used) > 3
+            for t in used[:6]:
+                for which, g in (('q', gq), ('eps', ge), ('rmin', gr)):
+                    d = systems.fd_type_gradient(s, spi, c[i], p[i], eps, rmin, which, t)
+                    # central differences with h = 1e-4 carry O((h/p)^2) error for
+                    # parameters p ~ 0.03: 2e-5 relative
+                    assert abs(g[t] - d) <= 2e-5 * max(abs(d), 1.0), (which, t, g[t], d)
+
+
+def test_adjoint_epsilon_euler_identity():
+    """E_LJ is homogeneous of degree 1 in the epsilons (pair epsilon =
+    sqrt(eps_a eps_b)), so sum_t eps_t dE/deps_t = E - E(eps = 0)."""
+    s, eps, rmin = _adjoint_case()
+    for spi in (1, 6):
+        c, p = systems.moderate_poses(s, spi, 3, seed=31 + spi)
+        for i in range(len(c)):
+            e, gx, gq, ge, gr = O.adjoint(s, spi, c[i], p[i], eps, rmin)
+            e_el = O.adjoint(s, spi, c[i], p[i], np.zeros_like(eps), rmin)[0]
+            assert abs(np.dot(eps, ge) - (e - e_el)) <= 1e-9 * max(1.0, abs(e - e_el))
