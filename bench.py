@@ -106,6 +106,65 @@ def parity_sample(system, nsteps):
     return float(err[mod].max()) if mod.any() else 0.0, agree, int(gm.sum())
 
 
+def adjoint_bench(npose, reps=5):
+    """Forward + backward of the pose-energy adjoint: npose poses (equal
+    shares of the 8 fragment types, uniform centres, Shoemake orientations)
+    against the 20,000-atom protein of the multi-fragment system (80 A box,
+    no water), timed with torch events on the current stream."""
+    import torch
+    from pygcmc_amd import adjoint, synth
+    from pygcmc_amd.engine import Engine
+    s = synth.multi_fragment(mcsteps=10)
+    eng = Engine()
+    eng.add_system(s)
+    eng.upload()
+    eps, rmin = adjoint.type_parameters(s)
+    rng = np.random.default_rng(4096)
+    T = 8
+    per = npose // T
+    batches = []
+    for spi in range(T):
+        F = s['frag'][spi]
+        nf = int(F['num_atoms'])
+        t = F['atoms'][:nf]['position'].astype(np.float64)
+        info = s['info'][0]
+        c = info['startxyz'].astype(np.float64) + rng.random((per, 3)) * info['cryst'].astype(np.float64)
+        u1, u2, u3 = rng.random((3, per))
+        q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                      np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], 1)
+        x, y, z, w = q.T
+        Rm = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                       2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                       2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1).reshape(per, 3, 3)
+        p = np.einsum('nij,aj->nai', Rm, t) + c[:, None, :]
+        batches.append((spi, torch.tensor(c, dtype=torch.float32, device='cuda'),
+                        torch.tensor(p, dtype=torch.float32, device='cuda')))
+
+    def once():
+        e = eps.clone().requires_grad_(True)
+        r = rmin.clone().requires_grad_(True)
+        for spi, c, p in batches:
+            pp = p.clone().requires_grad_(True)
+            E = adjoint.pose_energy(eng, 0, spi, c, pp, e, r)
+            E.sum().backward()
+        return e.grad, r.grad
+
+    once()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        once()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    eng.close()
+    nfix = int(s['frag'][0]['startRes'])
+    env = int(s['res'][nfix - 1]['atomStart'] + s['res'][nfix - 1]['atomNum'])
+    return {'poses': per * T, 'ms': round(ms, 3), 'poses_per_s': round(per * T / (ms / 1e3), 1),
+            'env_atoms': env, 'dtype': 'fp32 (energies fp64 from compensated fp32)'}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -118,6 +177,11 @@ def main():
     ap.add_argument('--mode', default='graph', choices=['batched', 'graph'])
     ap.add_argument('--cell-size', type=float, default=0.0)
     ap.add_argument('--wgs', type=int, default=0)
+    ap.add_argument('--wg-size', type=int, default=0)
+    ap.add_argument('--kernel-only', action='store_true',
+                    help='only the timed batched run (for profiler runs): no graph, plain, parity, '
+                         'adjoint or CPU legs')
+    ap.add_argument('--adjoint-poses', type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported in this build '
@@ -129,7 +193,7 @@ def main():
     systems, sizes = build_replicas(args.replicas, total)
     R = len(systems)
     log(f'built {R} systems ({int(sizes.sum())} fixed atoms) in {time.time() - t0:.1f}s')
-    tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs)
+    tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs, wg_size=args.wg_size)
 
     # batched form over the whole schedule
     eng = make_engine(copies(systems), tuning)
@@ -140,11 +204,19 @@ def main():
     ms_total = eng.run(args.steps, 'batched')
     ms_kernel = eng.last_kernel_ms()
     c1 = counters_sum(eng, R)
+    launch = eng.launch_info()
     eng.close()
     steps_done = R * args.steps
     value = steps_done / (ms_total / 1e3)
     evals, pairs, atoms, groups, fatoms = c1 - c0
     log(f'batched: {steps_done} steps in {ms_total:.1f} ms (kernel {ms_kernel:.1f} ms)')
+
+    tflops = FLOP_PER_PAIR * pairs / (ms_kernel / 1e3) / 1e12
+    gbs = BYTES_PER_ITEM * (atoms + groups + fatoms) / (ms_kernel / 1e3) / 1e9
+    if args.kernel_only:
+        print(json.dumps({'value': round(value, 1), 'unit': 'MC steps/s', 'kernel_ms': ms_kernel,
+                          'tflops': tflops, 'algorithmic_GBps': gbs, 'launch': launch}), flush=True)
+        return
 
     # graph replay of the same chains (timing only, from the warm state)
     graph_rate = None
@@ -181,8 +253,10 @@ def main():
     cpu_rate, cpu_done, cpu_dt = cpu_baseline(copies(states_w[:4]), args.warmup, args.cpu_seconds)
     log(f'cpu: {cpu_done} steps in {cpu_dt:.1f}s')
 
-    tflops = FLOP_PER_PAIR * pairs / (ms_kernel / 1e3) / 1e12
-    gbs = BYTES_PER_ITEM * (atoms + groups + fatoms) / (ms_kernel / 1e3) / 1e9
+    adj = adjoint_bench(args.adjoint_poses) if args.adjoint_poses > 0 else None
+    if adj:
+        log(f"adjoint: {adj['poses']} poses fwd+bwd in {adj['ms']:.2f} ms")
+
     out = {
         'metric': 'MC steps/sec + trial dE evals/sec, batched vs plain, at |dE| err <= 1e-4 kcal/mol vs FP64',
         'value': round(value, 1),
@@ -209,6 +283,8 @@ def main():
         'max_abs_dE_kJ': max_err,
         'max_abs_dE_kcal': max_err / 4.184,
         'decision_agreement': agree,
+        'launch': launch,
+        'adjoint': adj,
         'roofline': {'bound': 'mfma', 'pipe': 'fp32-valu', 'kernel': 'k_batched',
                      'achieved': round(tflops, 3), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': round(tflops / FP32_PEAK_TFLOPS, 4), 'traffic': None,

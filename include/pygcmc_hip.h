@@ -205,6 +205,24 @@ int gcmc_engine_adjoint_backward(gcmc_engine *e, int npose, int per_pose, int nc
 int gcmc_cavity_grid(int natom, const double *xyz, const double *startxyz, double grid_dx,
                      const int *grid_n, int max_points, float *grid, int *npoints);
 
+/* ---- smooth particle-mesh Ewald, reciprocal part ---------------------------
+ * No counterpart in the reference (its PME flag, gcmc/cpp/gcmc.h:91, is never
+ * read).  Orthorhombic box box[3] (A), grid[3] points (4..4096 each), Ewald
+ * splitting beta (1/A), order-4 B-splines, K = 1388.431112 kJ/mol A e^-2.
+ * Charge grid Q accumulates spread charges until cleared; solve computes
+ * E_rec = 1/2 sum_m G(m)|FFT(Q)(m)|^2 into *energy (a DEVICE double, may be
+ * NULL) and the potential grid; gather interpolates the reciprocal potential
+ * per unit charge (kJ/mol/e) at points.  xyzq [n][4] = (x, y, z, q) and xyz
+ * [n][3] are DEVICE float arrays; work is queued on `stream` (NULL = the
+ * default stream).  One plan must not be used from two streams at once. */
+typedef struct gcmc_pme gcmc_pme;
+int gcmc_pme_create(gcmc_pme **out, const int *grid, const float *box, float beta);
+int gcmc_pme_destroy(gcmc_pme *p);
+int gcmc_pme_clear(gcmc_pme *p, void *stream);
+int gcmc_pme_spread(gcmc_pme *p, int n, const float *xyzq, void *stream);
+int gcmc_pme_solve(gcmc_pme *p, double *energy, void *stream);
+int gcmc_pme_gather(gcmc_pme *p, int n, const float *xyz, float *phi, void *stream);
+
 /* Host execution of the engine's FP32 group-inclusion test -- the code the
  * GPU runs, compiled for the CPU -- for n pairs of points: out[i] = 1 when
  * distanceP(a_i, b_i, cryst) <= cutoff (gcmc/cpp/gcmc_energy.h:115-122). */

@@ -1,0 +1,135 @@
+"""FP64 Ewald sums -- the checker of the PME kernels.
+
+TEST INFRASTRUCTURE: only tests/ and bench.py's CPU leg use this file.
+
+The reference declares a PME flag (gcmc/cpp/gcmc.h:91, gcmc/python/values.py:55)
+that no kernel reads, so there is no reference algorithm to restate; this is
+the textbook Ewald decomposition (Ewald 1921; notation of Essmann et al.,
+J. Chem. Phys. 103, 8577 (1995)) of the Coulomb energy of point charges in an
+orthorhombic periodic box, in the units of calcEnergy
+(K = 1388.431112 kJ/mol A e^-2, gcmc/cpp/gcmc_energy.h:137):
+
+    E      = E_real + E_recip + E_self
+    E_real = K sum_{i<j, n} q_i q_j erfc(beta r) / r        (minimum image, r <= rc)
+    E_recip= K / (2 pi V) sum_{m != 0} exp(-pi^2 m^2 / beta^2) / m^2 |S(m)|^2
+    S(m)   = sum_j q_j exp(2 pi i m . x_j),  m = (k1/L1, k2/L2, k3/L3)
+    E_self = -K beta / sqrt(pi) sum_j q_j^2
+
+Pinned by: the NaCl Madelung constant, independence of the total from beta
+(tests/test_pme_oracle.py).
+"""
+
+import math
+
+import numpy as np
+from scipy.special import erfc
+
+KCOUL = 1388.431112
+
+
+def recip_energy(xyz, q, L, beta, kmax):
+    """Reciprocal-space Ewald energy with |k_i| <= kmax[i] (FP64)."""
+    xyz = np.asarray(xyz, np.float64)
+    q = np.asarray(q, np.float64)
+    L = np.asarray(L, np.float64)
+    V = float(np.prod(L))
+    k = [np.arange(-kmax[i], kmax[i] + 1) for i in range(3)]
+    K1, K2, K3 = np.meshgrid(*k, indexing='ij')
+    m = np.stack([K1.ravel() / L[0], K2.ravel() / L[1], K3.ravel() / L[2]], 1)
+    m2 = (m * m).sum(1)
+    keep = m2 > 0
+    m, m2 = m[keep], m2[keep]
+    e = 0.0
+    for c0 in range(0, len(m), 4096):
+        mm = m[c0:c0 + 4096]
+        ph = 2 * math.pi * xyz @ mm.T                       # [N, M]
+        S_re = (q[:, None] * np.cos(ph)).sum(0)
+        S_im = (q[:, None] * np.sin(ph)).sum(0)
+        w = np.exp(-math.pi ** 2 * m2[c0:c0 + 4096] / beta ** 2) / m2[c0:c0 + 4096]
+        e += float((w * (S_re ** 2 + S_im ** 2)).sum())
+    return KCOUL * e / (2 * math.pi * V)
+
+
+def real_energy(xyz, q, L, beta, rc):
+    """Real-space Ewald energy, minimum image, pairs with r <= rc (FP64)."""
+    xyz = np.asarray(xyz, np.float64)
+    q = np.asarray(q, np.float64)
+    L = np.asarray(L, np.float64)
+    e = 0.0
+    for i in range(len(q) - 1):
+        d = xyz[i + 1:] - xyz[i]
+        d -= np.round(d / L) * L
+        r = np.sqrt((d * d).sum(1))
+        m = r <= rc
+        e += float((q[i] * q[i + 1:][m] * erfc(beta * r[m]) / r[m]).sum())
+    return KCOUL * e
+
+
+def self_energy(q, beta):
+    q = np.asarray(q, np.float64)
+    return -KCOUL * beta / math.sqrt(math.pi) * float((q * q).sum())
+
+
+def ewald_energy(xyz, q, L, beta, rc, kmax):
+    return real_energy(xyz, q, L, beta, rc) + recip_energy(xyz, q, L, beta, kmax) + self_energy(q, beta)
+
+
+def recip_potential(xyz, q, L, beta, kmax, points):
+    """Reciprocal-space potential (kJ/mol/e) of the charges at `points`:
+    phi(r) = K / (pi V) sum_{m != 0} exp(-pi^2 m^2/beta^2)/m^2 Re(S(m) e^{-2 pi i m.r})."""
+    xyz = np.asarray(xyz, np.float64)
+    q = np.asarray(q, np.float64)
+    pts = np.asarray(points, np.float64).reshape(-1, 3)
+    L = np.asarray(L, np.float64)
+    V = float(np.prod(L))
+    k = [np.arange(-kmax[i], kmax[i] + 1) for i in range(3)]
+    K1, K2, K3 = np.meshgrid(*k, indexing='ij')
+    m = np.stack([K1.ravel() / L[0], K2.ravel() / L[1], K3.ravel() / L[2]], 1)
+    m2 = (m * m).sum(1)
+    keep = m2 > 0
+    m, m2 = m[keep], m2[keep]
+    out = np.zeros(len(pts))
+    for c0 in range(0, len(m), 4096):
+        mm = m[c0:c0 + 4096]
+        w = np.exp(-math.pi ** 2 * m2[c0:c0 + 4096] / beta ** 2) / m2[c0:c0 + 4096]
+        ph = 2 * math.pi * xyz @ mm.T
+        S_re = (q[:, None] * np.cos(ph)).sum(0)
+        S_im = (q[:, None] * np.sin(ph)).sum(0)
+        pp = 2 * math.pi * pts @ mm.T
+        out += (w[None, :] * (S_re[None, :] * np.cos(pp) + S_im[None, :] * np.sin(pp))).sum(1)
+    return KCOUL * out / (math.pi * V)
+
+
+def bspline4(u):
+    """Cardinal B-spline of order 4, M4(u), u in [0, 4)."""
+    u = np.asarray(u, np.float64)
+    out = np.zeros_like(u)
+    a = (u >= 0) & (u < 1)
+    out[a] = u[a] ** 3 / 6
+    b = (u >= 1) & (u < 2)
+    ub = u[b]
+    out[b] = (-3 * ub ** 3 + 12 * ub ** 2 - 12 * ub + 4) / 6
+    c = (u >= 2) & (u < 3)
+    uc = u[c]
+    out[c] = (3 * uc ** 3 - 24 * uc ** 2 + 60 * uc - 44) / 6
+    d = (u >= 3) & (u < 4)
+    out[d] = (4 - u[d]) ** 3 / 6
+    return out
+
+
+def nacl_crystal(ncell=2, a=5.64):
+    """Rock-salt lattice, ncell^3 conventional cells of edge a (A):
+    returns (xyz, q, L)."""
+    base = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    pts, qs = [], []
+    for i in range(ncell):
+        for j in range(ncell):
+            for k in range(ncell):
+                o = np.array([i, j, k], np.float64)
+                for b in base:
+                    pts.append((o + b) * a)
+                    qs.append(1.0)
+                    pts.append((o + b + np.array([.5, 0, 0])) * a)
+                    qs.append(-1.0)
+    L = [ncell * a] * 3
+    return np.array(pts), np.array(qs), L

@@ -24,7 +24,7 @@ MODULE = os.path.join(HERE, 'gpu' + EXT_SUFFIX)
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('PYGCMC_ARCH', 'gfx950')
 
-HIP_SOURCES = ['engine.hip', 'grid.hip']
+HIP_SOURCES = ['engine.hip', 'grid.hip', 'pme.hip']
 
 HIP_FLAGS = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
@@ -49,7 +49,9 @@ def build_library():
         _run([HIPCC, *HIP_FLAGS, '-I', os.path.join(ROOT, 'include'), '-c',
               os.path.join(CSRC, src), '-o', obj])
         objs.append(obj)
-    _run([HIPCC, '-shared', f'--offload-arch={ARCH}', *objs, '-o', LIB + '.tmp'])
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), 'lib')
+    _run([HIPCC, '-shared', f'--offload-arch={ARCH}', *objs, '-L', rocm_lib, '-lhipfft',
+          f'-Wl,-rpath,{rocm_lib}', '-o', LIB + '.tmp'])
     os.replace(LIB + '.tmp', LIB)
 
 

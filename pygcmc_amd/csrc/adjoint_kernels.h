@@ -27,7 +27,8 @@
 
 namespace gcmc {
 
-constexpr float SIGMA_PER_RMINHALF = 0.17817974362806785f;  // 2^(5/6) / 10
+constexpr double SIGMA_PER_RMINHALF_D = 0.17817974362806785;  // 2^(5/6) / 10
+constexpr float SIGMA_PER_RMINHALF = (float)SIGMA_PER_RMINHALF_D;
 
 struct AdjLds {
     float2 *tab;    // [X*Y] (100 sigma^2, 4 epsilon) from the per-type parameters
@@ -150,21 +151,19 @@ __global__ __launch_bounds__(NT) void k_adjoint(const Rep *reps, int r, int spi,
     P.A = adj_carve(smem + adj_off, X, Y);
     const Species &F = C.sp[spi];
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
-    // per-type sigma (nm) and epsilon (kJ/mol), staged in gq / ge
     for (int t = tid; t < Y; t += NT) {
         const float ek = fabsf(eps_kcal[t]);
         P.A.inv2e[t] = ek > 0.0f ? 0.5f / ek : 0.0f;
         P.A.dq[t] = dq ? dq[t] : 0.0f;
-        P.A.gq[t] = rmin_half[t] * SIGMA_PER_RMINHALF;
-        P.A.ge[t] = 4.184f * ek;
     }
     for (int q = tid; q < 6 * MAXF; q += NT) P.A.red[q] = 0.0f;
     load_species_table(C, L);
-    __syncthreads();
+    // pair entries in FP64, rounded once to FP32 (as a host-built table is)
     for (int q = tid; q < X * Y; q += NT) {
         const int x = q / Y, y = q - x * Y;
-        const float sg = 0.5f * (P.A.gq[x] + P.A.gq[y]);
-        P.A.tab[q] = make_float2(100.0f * sg * sg, 4.0f * sqrtf(P.A.ge[x] * P.A.ge[y]));
+        const double sg = 0.5 * ((double)rmin_half[x] + (double)rmin_half[y]) * (double)SIGMA_PER_RMINHALF_D;
+        const double ee = 4.184 * 4.184 * fabs((double)eps_kcal[x]) * fabs((double)eps_kcal[y]);
+        P.A.tab[q] = make_float2((float)(100.0 * sg * sg), (float)(4.0 * sqrt(ee)));
     }
     if (tid < nf) {
         const float *p = poses + ((size_t)i * nf + tid) * 3;
