@@ -182,6 +182,7 @@ def main():
                     help='only the timed batched run (for profiler runs): no graph, plain, parity, '
                          'adjoint or CPU legs')
     ap.add_argument('--adjoint-poses', type=int, default=4096)
+    ap.add_argument('--autotune', type=int, default=1, help='pick cell/workgroup sizes by timing short runs')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported in this build '
@@ -194,6 +195,12 @@ def main():
     R = len(systems)
     log(f'built {R} systems ({int(sizes.sum())} fixed atoms) in {time.time() - t0:.1f}s')
     tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs, wg_size=args.wg_size)
+    tune_table = None
+    if args.autotune:
+        from pygcmc_amd.autotune import autotune
+        t1 = time.time()
+        tuning, tune_table = autotune(systems, log=log)
+        log(f'autotune: {tuning} in {time.time() - t1:.1f}s')
 
     # batched form over the whole schedule
     eng = make_engine(copies(systems), tuning)
@@ -284,6 +291,8 @@ def main():
         'max_abs_dE_kcal': max_err / 4.184,
         'decision_agreement': agree,
         'launch': launch,
+        'tuning': tuning,
+        'autotune_ms': None if tune_table is None else [[t, round(ms, 3)] for t, ms in tune_table],
         'adjoint': adj,
         'roofline': {'bound': 'mfma', 'pipe': 'fp32-valu', 'kernel': 'k_batched',
                      'achieved': round(tflops, 3), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',

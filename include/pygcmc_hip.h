@@ -192,6 +192,22 @@ int gcmc_engine_adjoint_backward(gcmc_engine *e, int npose, int per_pose, int nc
                                  const float *dEdx, const float *partial, const float *weights,
                                  float *grad_pose, float *grad_cols, void *stream);
 
+/* ---- Ewald electrostatics (no counterpart in the reference) ---------------
+ * Before upload: give replica `replica` Ewald electrostatics with splitting
+ * beta (1/A) and a PME grid of about grid_spacing A (order-4 B-splines).
+ * Its trial energies then use K q q' erfc(beta r)/r in the group-cutoff pair
+ * terms plus the reciprocal interaction sum_f q_f phi(x_f) with the PME
+ * potential phi of the current state (the moving molecule's own share
+ * removed), and the potential is updated after every move.  Such replicas
+ * run in GCMC_MODE_PLAIN only.  The reference ignores InfoStruct.PME, and so
+ * do runGCMC_hip and gcmc_engine_add_system.  eval_poses adds the same
+ * reciprocal term; its exclude[i], when a molecule of the evaluated species,
+ * must be one present in the state. */
+int gcmc_engine_set_ewald(gcmc_engine *e, int replica, float beta, float grid_spacing);
+/* Reciprocal potential (kJ/mol/e) of the replica's current state at n points
+ * xyz[n][3] (host arrays). */
+int gcmc_engine_ewald_potential(gcmc_engine *e, int replica, int n, const float *xyz, float *phi);
+
 /* ---- cavity-bias grid (gcmc/python/dataset.py:21-139) --------------------
  * Occupancy of the grid_n[0] x grid_n[1] x grid_n[2] cells of edge grid_dx
  * starting at startxyz by the natom atoms xyz[natom][3] (FP64, host memory):

@@ -133,3 +133,92 @@ def nacl_crystal(ncell=2, a=5.64):
                     qs.append(-1.0)
     L = [ncell * a] * 3
     return np.array(pts), np.array(qs), L
+
+
+class EwaldEnv:
+    """Reciprocal Ewald potential of fixed charges, structure factors computed
+    once: phi(points) = K/(pi V) sum_m w(m) Re(S(m) e^{-2 pi i m.r})."""
+
+    def __init__(self, xyz, q, L, beta, kmax):
+        xyz = np.asarray(xyz, np.float64)
+        q = np.asarray(q, np.float64)
+        L = np.asarray(L, np.float64)
+        self.V = float(np.prod(L))
+        k = [np.arange(-kmax[i], kmax[i] + 1) for i in range(3)]
+        K1, K2, K3 = np.meshgrid(*k, indexing='ij')
+        m = np.stack([K1.ravel() / L[0], K2.ravel() / L[1], K3.ravel() / L[2]], 1)
+        m2 = (m * m).sum(1)
+        keep = m2 > 0
+        self.m = m[keep]
+        m2 = m2[keep]
+        self.w = np.exp(-math.pi ** 2 * m2 / beta ** 2) / m2
+        self.S_re = np.zeros(len(self.m))
+        self.S_im = np.zeros(len(self.m))
+        for c0 in range(0, len(q), 256):
+            ph = 2 * math.pi * xyz[c0:c0 + 256] @ self.m.T
+            self.S_re += (q[c0:c0 + 256, None] * np.cos(ph)).sum(0)
+            self.S_im += (q[c0:c0 + 256, None] * np.sin(ph)).sum(0)
+
+    def phi(self, points):
+        pts = np.asarray(points, np.float64).reshape(-1, 3)
+        pp = 2 * math.pi * pts @ self.m.T
+        return KCOUL / (math.pi * self.V) * (self.w[None, :] * (self.S_re[None, :] * np.cos(pp) +
+                                                                self.S_im[None, :] * np.sin(pp))).sum(1)
+
+
+def present_groups(sysd, exclude=-1):
+    """Residue indices in the state: fixed groups, then each species' active
+    slots (calcEnergy's order, gcmc/cpp/gcmc_energy.h:152-191), minus exclude."""
+    frag = sysd['frag']
+    out = list(range(int(frag[0]['startRes'])))
+    for F in frag:
+        out += list(range(int(F['startRes']), int(F['startRes']) + int(F['totalNum'])))
+    return [g for g in out if g != exclude]
+
+
+def env_charges(sysd, exclude=-1):
+    res, atoms = sysd['res'], sysd['atoms']
+    idx = np.concatenate([np.arange(res[g]['atomStart'], res[g]['atomStart'] + res[g]['atomNum'])
+                          for g in present_groups(sysd, exclude)])
+    return atoms['position'][idx].astype(np.float64), atoms['charge'][idx].astype(np.float64)
+
+
+def pose_energy_ewald(sysd, species, centre, pose, beta, env, exclude=-1):
+    """FP64 trial energy in Ewald mode: LJ + K q q' erfc(beta r)/r over the
+    groups passing calcEnergy's FP32 group-centre test, plus sum_f q_f
+    phi_env(x_f) with env an EwaldEnv of every present charge but the
+    excluded molecule's."""
+    from oracle import oracle as O
+    info = sysd['info'][0]
+    L = np.asarray(info['cryst'], np.float64)
+    Lf = np.asarray(info['cryst'], np.float32)
+    rc = np.float32(info['cutoff'])
+    Y = int(info['ffYNum'])
+    ff = sysd['ff'].astype(np.float64)
+    F = sysd['frag'][species]
+    nf = int(F['num_atoms'])
+    qf = F['atoms']['charge'][:nf].astype(np.float64)
+    tf = F['atoms']['type'][:nf]
+    P = np.asarray(pose, np.float32).astype(np.float64)
+    c32 = np.asarray(centre, np.float32)
+    e = 0.0
+    res, atoms = sysd['res'], sysd['atoms']
+    for g in present_groups(sysd, exclude):
+        if not O.distanceP32(c32, res[g]['position'], Lf) <= rc:
+            continue
+        a0, na = int(res[g]['atomStart']), int(res[g]['atomNum'])
+        A = atoms[a0:a0 + na]
+        xa = A['position'].astype(np.float64)
+        for f in range(nf):
+            d = xa - P[f]
+            d -= np.where(d / L >= 0, np.floor(d / L + 0.5), np.ceil(d / L - 0.5)) * L
+            r2 = (d * d).sum(1)
+            r = np.sqrt(r2)
+            ti = tf[f] * Y + A['type']
+            sig, eps = ff[2 * ti], ff[2 * ti + 1]
+            s2 = sig * sig * 100.0 / r2
+            s6 = s2 ** 3
+            e += float((4 * eps * (s6 * s6 - s6)).sum())
+            e += float((KCOUL * qf[f] * A['charge'].astype(np.float64) * erfc(beta * r) / r).sum())
+    e += float((qf * env.phi(P)).sum())
+    return e

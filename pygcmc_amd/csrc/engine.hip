@@ -113,9 +113,20 @@ struct HostRep {
     DevBuf trbuf;
     int maxk = 1;
     int host_step = 0;
+    float ew_beta = 0.0f;      // > 0: Ewald electrostatics with PME
+    float ew_spacing = 0.0f;
+    gcmc_pme *pme = nullptr;
 };
 
 }  // namespace
+
+// PME plan internals used by the Ewald mode (pme.hip)
+namespace gcmc_internal {
+const float *pme_phi(const gcmc_pme *p);
+const float *pme_kernel(const gcmc_pme *p);
+void pme_grid(const gcmc_pme *p, int K[3]);
+int pme_spread_counted(gcmc_pme *p, int cap, const int *count, const float4 *xyzq, int *reset, hipStream_t s);
+}  // namespace gcmc_internal
 
 struct gcmc_engine {
     gcmc_tuning tune{};
@@ -407,6 +418,43 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     HIPX(B.alloc(&R.E, 2 * (size_t)H.maxk));
     HIPX(B.alloc(&R.pose, (size_t)H.maxk * MAXF));
     HIPX(B.alloc(&R.adj, (size_t)H.maxk * ADJW));
+    if (H.ew_beta > 0.0f) {
+        // Ewald mode: PME plan over the box, charge-delta list, and the
+        // reciprocal potential of the initial state (fixed atoms and the
+        // molecules present)
+        int K[3];
+        for (int i = 0; i < 3; ++i) K[i] = std::max(4, (int)std::lround(I.cryst[i] / H.ew_spacing));
+        int rc = gcmc_pme_create(&H.pme, K, I.cryst, H.ew_beta);
+        if (rc) return rc;
+        R.ew = 1;
+        R.ew_beta = H.ew_beta;
+        for (int i = 0; i < 3; ++i) R.pk[i] = K[i];
+        R.pme_phi = gcmc_internal::pme_phi(H.pme);
+        R.pme_gk = gcmc_internal::pme_kernel(H.pme);
+        R.pme_cap = 2 * H.maxk * MAXF;
+        HIPX(B.alloc(&R.pme_delta, (size_t)R.pme_cap));
+        std::vector<int> zero1(1, 0);
+        HIPX(put(&R.pme_ndelta, B, zero1.data(), 1));
+        std::vector<float4> q;
+        const int nfix = H.frag[0].startRes;
+        auto add_res = [&](int g) {
+            const residue &rr = H.res[g];
+            for (int a = rr.atomStart; a < rr.atomStart + rr.atomNum; ++a) {
+                const Atom &A = H.atoms[a];
+                q.push_back(make_float4(A.position[0], A.position[1], A.position[2], A.charge));
+            }
+        };
+        for (int g = 0; g < nfix; ++g) add_res(g);
+        for (int sp = 0; sp < T; ++sp)
+            for (int j = 0; j < H.frag[sp].totalNum; ++j) add_res(H.frag[sp].startRes + j);
+        float4 *dq = nullptr;
+        HIPX(put(&dq, B, q.data(), q.size()));
+        rc = gcmc_pme_clear(H.pme, E->stream);
+        if (!rc) rc = gcmc_pme_spread(H.pme, (int)q.size(), (const float *)dq, E->stream);
+        if (!rc) rc = gcmc_pme_solve(H.pme, nullptr, E->stream);
+        if (rc) return rc;
+        HIPX(hipStreamSynchronize(E->stream));
+    }
     return 0;
 }
 
@@ -501,7 +549,16 @@ void launch_plain(gcmc_engine *E, int nsteps)
                            E->d_sched, E->lay, r, 0);
         const int rem = E->reps[r]->rep.nmoves - E->reps[r]->host_step;
         const int n = std::min(nsteps, std::max(rem, 0));
-        for (int s = 0; s < n; ++s) launch_plain_step<NT, S>(E, r);
+        HostRep &H = *E->reps[r];
+        for (int s = 0; s < n; ++s) {
+            launch_plain_step<NT, S>(E, r);
+            if (H.pme) {
+                // charges moved by the step enter the PME grid; new potential
+                gcmc_internal::pme_spread_counted(H.pme, H.rep.pme_cap, H.rep.pme_ndelta, H.rep.pme_delta,
+                                                  H.rep.pme_ndelta, E->stream);
+                gcmc_pme_solve(H.pme, nullptr, E->stream);
+            }
+        }
     }
 }
 
@@ -629,6 +686,7 @@ int gcmc_engine_destroy(gcmc_engine *E)
     if (E->gexec) hipGraphExecDestroy(E->gexec);
     if (E->graph) hipGraphDestroy(E->graph);
     for (HostRep *H : E->reps) {
+        if (H->pme) gcmc_pme_destroy(H->pme);
         H->buf.release();
         H->trbuf.release();
         delete H;
@@ -722,6 +780,9 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
     if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
     if (nsteps < 0) return fail(GCMC_E_ARG, "negative step count");
     if (nsteps == 0) return 0;
+    if (mode != GCMC_MODE_PLAIN)
+        for (HostRep *H : E->reps)
+            if (H->pme) return fail(GCMC_E_ARG, "systems with Ewald electrostatics run in plain mode");
     HIPX(hipEventRecord(E->ev0, E->stream));
     if (mode == GCMC_MODE_PLAIN) {
         dispatch<RunPlain>(E, nsteps);
@@ -1008,6 +1069,42 @@ int gcmc_engine_eval_poses(gcmc_engine *E, int r, int species, int npose, const 
     return 0;
 }
 
+int gcmc_engine_set_ewald(gcmc_engine *E, int r, float beta, float grid_spacing)
+{
+    if (!E) return fail(GCMC_E_ARG, "null engine");
+    if (E->uploaded) return fail(GCMC_E_STATE, "set Ewald parameters before upload()");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    if (!(beta > 0.0f) || !(grid_spacing > 0.0f)) return fail(GCMC_E_ARG, "beta and grid spacing must be positive");
+    E->reps[r]->ew_beta = beta;
+    E->reps[r]->ew_spacing = grid_spacing;
+    return 0;
+}
+
+int gcmc_engine_ewald_potential(gcmc_engine *E, int r, int n, const float *xyz, float *phi)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    if (!E->reps[r]->pme) return fail(GCMC_E_STATE, "replica has no Ewald electrostatics");
+    if (n < 0 || (n > 0 && (!xyz || !phi))) return fail(GCMC_E_ARG, "bad arguments");
+    if (n == 0) return 0;
+    DevBuf B;
+    float *dx = nullptr, *dp = nullptr;
+    hipError_t e = put(&dx, B, xyz, 3 * (size_t)n);
+    if (e == hipSuccess) e = B.alloc(&dp, (size_t)n);
+    if (e != hipSuccess) {
+        B.release();
+        return fail(GCMC_E_HIP, std::string("ewald_potential: ") + hipGetErrorString(e));
+    }
+    int rc = gcmc_pme_gather(E->reps[r]->pme, n, dx, dp, E->stream);
+    if (!rc) {
+        e = hipStreamSynchronize(E->stream);
+        if (e == hipSuccess) e = hipMemcpy(phi, dp, n * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(GCMC_E_HIP, std::string("ewald_potential: ") + hipGetErrorString(e));
+    }
+    B.release();
+    return rc;
+}
+
 int gcmc_engine_adjoint(gcmc_engine *E, int r, int species, int npose, const float *centres,
                         const float *poses, const float *eps_kcal, const float *rmin_half, const float *dq,
                         double *energy, float *dEdx, float *partial, void *stream)
@@ -1020,6 +1117,7 @@ int gcmc_engine_adjoint(gcmc_engine *E, int r, int species, int npose, const flo
     if (npose == 0) return 0;
     if (!centres || !poses || !eps_kcal || !rmin_half || !energy || !dEdx || !partial)
         return fail(GCMC_E_ARG, "null array");
+    if (H.pme) return fail(GCMC_E_ARG, "the adjoint covers plain Coulomb, not Ewald electrostatics");
     const int nf = H.frag[species].num_atoms;
     const int X = H.rep.X, Y = H.rep.Y;
     const int adj_off = (E->lay.total + 15) & ~15;

@@ -17,6 +17,7 @@
 #pragma once
 
 #include "engine_types.h"
+#include "pme_spline.h"
 
 namespace gcmc {
 
@@ -153,6 +154,8 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
     const float2 *PT = STAGED ? L.pair : C.pair;
     const float Lx = C.L[0], Ly = C.L[1], Lz = C.L[2];
     const float ix = C.invL[0], iy = C.invL[1], iz = C.invL[2];
+    const bool ew = C.ew != 0;
+    const float eb = C.ew_beta;
     for (int t = threadIdx.x; t < A; t += NT) {
         const int g = bsearch_le(L.incn, n, t);
         const int a = L.incs[g] + (t - L.incn[g]);
@@ -179,7 +182,8 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
             const float s2 = pp.x * ir2;
             const float s6 = s2 * s2 * s2;
             const float lj = pp.y * fmaf(s6, s6, -s6);     // 4 eps (s^12 - s^6)
-            const float el = (fr.w * at.w) * ir;            // K q_f q_a / r
+            float el = (fr.w * at.w) * ir;                  // K q_f q_a / r
+            if (ew) el *= erfcf(eb * (r2 * ir));            // Ewald real-space part
             const float term = lj + el;
             const float s = acc + term;                     // TwoSum
             const float bp = s - acc;
@@ -392,6 +396,47 @@ __device__ inline void count_add(T *p, T v)
     else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ---- Ewald reciprocal term of a pose ------------------------------------------
+// sum_f q_f phi(x_f) over the pose held in L.frag, phi the PME potential of the
+// current state.  When the molecule itself is in that state (base >= 0: its
+// atoms y_g at C.atom[base..]), its own share of phi -- what the grid holds
+// for its charges, periodic images and neutralising background included --
+// is removed: sum_{f,g} q_f q_g Kgrid(x_f, y_g) (pme_spline.h).  What remains
+// is the reciprocal interaction with every other charge; for a rigid
+// molecule the Ewald self, intramolecular and self-image terms are constants
+// of the species, left out alike on insertion and deletion.  Called by every
+// thread; thread 0 gets the sum.
+template <int NT>
+__device__ inline double ewald_recip(const Rep &C, const Species &F, const Lds &L, int nf, int base)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double v = 0.0;
+    if (tid < nf) {
+        const float4 x = L.frag[tid];
+        v = (double)F.tmpl[tid].w * (double)gcmc_spline::gather(C.pme_phi, C.pk, C.L, x.x, x.y, x.z);
+    }
+    if (base >= 0) {
+        const int nw = nf * nf * 7;
+        for (int w = tid; w < nw; w += NT) {
+            const int f = w / (7 * nf), g = (w / 7) % nf, j0 = w % 7;
+            const float4 xf = L.frag[f];
+            const float4 yg = C.atom[base + g];
+            const float xv[3] = {xf.x, xf.y, xf.z}, yv[3] = {yg.x, yg.y, yg.z};
+            const float kv = gcmc_spline::pair_kernel_slab(C.pme_gk, C.pk, C.L, xv, yv, j0);
+            v -= (double)F.tmpl[f].w * (double)F.tmpl[g].w * (double)kv;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) L.dred[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (tid == 0)
+        for (int q = 0; q < NT / 64; ++q) t += L.dred[q];
+    __syncthreads();
+    return t;
+}
+
 // ---- one trial evaluation ----------------------------------------------------
 // item < k: trial item (insert/delete) or E_old (translate/rotate);
 // item >= k: E_new of trial item-k (translate/rotate).
@@ -495,6 +540,11 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
     unsigned long long ticks[4];
     ticks[0] = wall_clock64() - tq0;
     eval_energy<NT, STAGED>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat, ticks);
+    if (C.ew) {
+        // the moving molecule (delete, translate, rotate) is in the PME grid
+        const double er = ewald_recip<NT>(C, F, L, nf, kind == K_INS ? -1 : F.atomBase + slot * nf);
+        if (tid == 0) E += er;
+    }
     if (tid == 0) {
         C.E[phase * k + t] = E;
         State &W = const_cast<State &>(S);
@@ -743,6 +793,16 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
             const int newN = N - 1;
             const int hole_slot = C.tslot[0];
             const int hole = F.startRes + hole_slot, last = F.startRes + newN;
+            if (C.ew) {
+                // the deleted molecule's charges leave the PME grid (the moved
+                // last molecule keeps its coordinates)
+                for (int a = tid; a < nf; a += NT) {
+                    const float4 o = C.atom[F.atomBase + hole_slot * nf + a];
+                    C.pme_delta[a] = make_float4(o.x, o.y, o.z, -F.tmpl[a].w);
+                }
+                if (tid == 0) *C.pme_ndelta = nf;
+                __syncthreads();
+            }
             if (newN > 0 && hole != last) {
                 const int bh = F.atomBase + hole_slot * nf, bl = F.atomBase + newN * nf;
                 for (int a = tid; a < nf; a += NT) {
@@ -781,10 +841,16 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
                 C.atype[base + a] = F.ttype[a];
             } else {
                 dst = C.atom[base + a];
+                if (C.ew) C.pme_delta[2 * idx] = make_float4(dst.x, dst.y, dst.z, -F.tmpl[a].w);
                 dst.x = p.x; dst.y = p.y; dst.z = p.z;
+            }
+            if (C.ew) {
+                if (kind == K_INS) C.pme_delta[2 * idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+                C.pme_delta[2 * idx + 1] = make_float4(p.x, p.y, p.z, F.tmpl[a].w);
             }
             C.atom[base + a] = dst;
         }
+        if (C.ew && tid == 0) *C.pme_ndelta = 2 * na * nf;
         if (tid == 0) {
             for (int q = 0; q < na; ++q) {
                 const int2 ts = L.accl[q];
@@ -1143,6 +1209,12 @@ __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int s
     uint32_t h;
     eval_energy<NT, STAGED>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n,
                             &h, &na);
+    if (C.ew) {
+        const int x = excl[i];
+        const bool own = x >= F.startRes && x < F.startRes + F.maxNum;
+        const double er = ewald_recip<NT>(C, F, L, nf, own ? F.atomBase + (x - F.startRes) * nf : -1);
+        if (tid == 0) e += er;
+    }
     if (tid == 0) {
         E[i] = e;
         ninc[i] = n;

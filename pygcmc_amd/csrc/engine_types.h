@@ -83,6 +83,15 @@ struct Rep {
     double *E;               // [2*maxk] E (phase 0), E_new (phase 1)
     float4 *pose;            // [maxk*MAXF] new poses
     uint32_t *adj;           // [maxk*ADJW] trial-centre adjacency rows
+    // Ewald electrostatics (ew != 0): pair terms K q q' erfc(beta r)/r and the
+    // reciprocal potential pme_phi [pk[0]][pk[1]][pk[2]] of the current state
+    int ew, pk[3];
+    float ew_beta;
+    int pme_cap;             // capacity of pme_delta
+    const float *pme_phi;
+    const float *pme_gk;     // grid kernel: potential grid of a unit charge at point 0
+    float4 *pme_delta;       // charge changes of the last move (x, y, z, dq)
+    int *pme_ndelta;
     DevTrace tr;
 };
 

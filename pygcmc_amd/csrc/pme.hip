@@ -12,8 +12,13 @@
 //   E_rec  = 1/2 sum_k Q(k) Phi(k) = 1/2 sum_m G(m) |FFT(Q)(m)|^2
 //   phi(r) = sum_k W_r(k) Phi(k)      reciprocal potential at r (per unit charge)
 //
+// The charge grid accumulates in 32.32 fixed point (int64 atomics): sums are
+// exact and order independent, so a run is reproducible and removing a
+// charge (spreading -q at the same point) cancels it exactly.
+//
 // Kernels (HBM / atomic bound; no MFMA -- not a dense contraction):
-//   k_pme_spread    one thread per charge, 64 FP32 atomic adds to Q
+//   k_pme_spread    one thread per charge, 64 integer atomic adds to Q
+//   k_pme_tofloat   fixed-point grid -> FP32 grid for the FFT
 //   k_pme_convolve  one thread per half-spectrum entry: energy term
 //                   (FP64 block sums) and X *= G
 //   k_pme_gather    one thread per point, 64 reads of Phi
@@ -26,6 +31,7 @@
 #include <vector>
 
 #include "../../include/pygcmc_hip.h"
+#include "pme_spline.h"
 
 namespace gcmc_internal {
 int set_error(int code, const std::string &msg);
@@ -40,40 +46,16 @@ struct Geo {
     float L[3];
 };
 
-// weights of the 4 grid points f, f-1, f-2, f-3 of scaled coordinate u
-__device__ inline void spline4(float u, int K, int &f, float w[4])
-{
-    const float fl = floorf(u);
-    f = (int)fl;
-    const float t = u - fl;                 // [0, 1)
-    const float t2 = t * t, t3 = t2 * t;
-    const float s = 1.0f - t;
-    w[0] = t3 * (1.0f / 6.0f);                                        // M4(t)
-    w[1] = (-3.0f * t3 + 3.0f * t2 + 3.0f * t + 1.0f) * (1.0f / 6.0f); // M4(t+1)
-    w[2] = (3.0f * t3 - 6.0f * t2 + 4.0f) * (1.0f / 6.0f);             // M4(t+2)
-    w[3] = s * s * s * (1.0f / 6.0f);                                  // M4(t+3)
-    f = ((f % K) + K) % K;
-}
+using gcmc_spline::spline4;
+using gcmc_spline::scaled;
 
-// scaled coordinate K * frac(x / L) in [0, K)
-__device__ inline float scaled(float x, float L, int K)
-{
-    float fr = x / L;
-    fr -= floorf(fr);
-    float u = fr * (float)K;
-    return u >= (float)K ? 0.0f : u;
-}
+constexpr double FIX = 4294967296.0;   // 2^32
 
-__global__ void k_pme_fill(float *p, size_t n)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
-}
-
-__global__ void k_pme_spread(int n, const float4 *xyzq, Geo g, float *Q)
+// n charges, or *count of them when count is not null (n = capacity)
+__global__ void k_pme_spread(int n, const int *count, const float4 *xyzq, Geo g, unsigned long long *Q)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (count && i >= *count)) return;
     const float4 a = xyzq[i];
     if (a.w == 0.0f) return;
     int f0, f1, f2;
@@ -88,14 +70,32 @@ __global__ void k_pme_spread(int n, const float4 *xyzq, Geo g, float *Q)
             int k1 = f1 - i1;
             k1 += k1 < 0 ? g.K[1] : 0;
             const float w01 = a.w * w0[i0] * w1[i1];
-            float *row = Q + ((size_t)k0 * g.K[1] + k1) * g.K[2];
+            unsigned long long *row = Q + ((size_t)k0 * g.K[1] + k1) * g.K[2];
             for (int i2 = 0; i2 < 4; ++i2) {
                 int k2 = f2 - i2;
                 k2 += k2 < 0 ? g.K[2] : 0;
-                atomicAdd(row + k2, w01 * w2[i2]);
+                const long long v = __double2ll_rn((double)(w01 * w2[i2]) * FIX);
+                atomicAdd(row + k2, (unsigned long long)v);
             }
         }
     }
+}
+
+__global__ void k_pme_tofloat(const unsigned long long *Qi, float *Q, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) Q[i] = (float)((double)(long long)Qi[i] * (1.0 / FIX));
+}
+
+__global__ void k_pme_zero64(unsigned long long *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_pme_zero_count(int *c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
 }
 
 // X: K0 x K1 x (K2/2+1) half spectrum; G same layout (real)
@@ -127,29 +127,7 @@ __global__ void k_pme_gather(int n, const float *xyz, Geo g, const float *Phi, f
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int f0, f1, f2;
-    float w0[4], w1[4], w2[4];
-    spline4(scaled(xyz[3 * i + 0], g.L[0], g.K[0]), g.K[0], f0, w0);
-    spline4(scaled(xyz[3 * i + 1], g.L[1], g.K[1]), g.K[1], f1, w1);
-    spline4(scaled(xyz[3 * i + 2], g.L[2], g.K[2]), g.K[2], f2, w2);
-    float s = 0.0f;
-    for (int i0 = 0; i0 < 4; ++i0) {
-        int k0 = f0 - i0;
-        k0 += k0 < 0 ? g.K[0] : 0;
-        for (int i1 = 0; i1 < 4; ++i1) {
-            int k1 = f1 - i1;
-            k1 += k1 < 0 ? g.K[1] : 0;
-            const float *row = Phi + ((size_t)k0 * g.K[1] + k1) * g.K[2];
-            float r = 0.0f;
-            for (int i2 = 0; i2 < 4; ++i2) {
-                int k2 = f2 - i2;
-                k2 += k2 < 0 ? g.K[2] : 0;
-                r = fmaf(w2[i2], row[k2], r);
-            }
-            s = fmaf(w0[i0] * w1[i1], r, s);
-        }
-    }
-    phi[i] = s;
+    phi[i] = gcmc_spline::gather(Phi, g.K, g.L, xyz[3 * i + 0], xyz[3 * i + 1], xyz[3 * i + 2]);
 }
 
 __global__ void k_zero_d(double *p)
@@ -163,7 +141,9 @@ struct gcmc_pme {
     gcmc_pme_k::Geo geo{};
     float beta = 0.0f;
     size_t ngrid = 0, nhalf = 0;
+    unsigned long long *Qi = nullptr;
     float *Q = nullptr, *Phi = nullptr, *G = nullptr;
+    float *Gk = nullptr;   // real-space grid kernel IFFT(G): Phi = Gk (*) Q
     hipfftComplex *X = nullptr;
     hipfftHandle fwd = 0, inv = 0;
     bool have_fwd = false, have_inv = false;
@@ -205,7 +185,9 @@ int gcmc_pme_destroy(gcmc_pme *p)
     if (!p) return 0;
     if (p->have_fwd) hipfftDestroy(p->fwd);
     if (p->have_inv) hipfftDestroy(p->inv);
+    if (p->Qi) (void)hipFree(p->Qi);
     if (p->Q) (void)hipFree(p->Q);
+    if (p->Gk) (void)hipFree(p->Gk);
     if (p->Phi) (void)hipFree(p->Phi);
     if (p->G) (void)hipFree(p->G);
     if (p->X) (void)hipFree(p->X);
@@ -234,7 +216,9 @@ int gcmc_pme_create(gcmc_pme **out, const int *grid, const float *box, float bet
     const int K0 = grid[0], K1 = grid[1], K2 = grid[2], K2h = K2 / 2 + 1;
     p->ngrid = (size_t)K0 * K1 * K2;
     p->nhalf = (size_t)K0 * K1 * K2h;
-    hipError_t e = hipMalloc(&p->Q, p->ngrid * sizeof(float));
+    hipError_t e = hipMalloc(&p->Qi, p->ngrid * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&p->Q, p->ngrid * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&p->Gk, p->ngrid * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&p->Phi, p->ngrid * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&p->G, p->nhalf * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&p->X, p->nhalf * sizeof(hipfftComplex));
@@ -280,6 +264,23 @@ int gcmc_pme_create(gcmc_pme **out, const int *grid, const float *box, float bet
         return pme_fft(r, "PME inverse plan");
     }
     p->have_inv = true;
+    // grid kernel: the potential grid of a unit charge at grid point 0
+    std::vector<hipfftComplex> Gc(p->nhalf);
+    for (size_t i = 0; i < p->nhalf; ++i) {
+        Gc[i].x = G[i];
+        Gc[i].y = 0.0f;
+    }
+    e = hipMemcpy(p->X, Gc.data(), p->nhalf * sizeof(hipfftComplex), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        gcmc_pme_destroy(p);
+        return pme_hip(e, "PME grid kernel");
+    }
+    r = hipfftExecC2R(p->inv, p->X, p->Gk);
+    if (r == HIPFFT_SUCCESS && hipDeviceSynchronize() != hipSuccess) r = HIPFFT_EXEC_FAILED;
+    if (r != HIPFFT_SUCCESS) {
+        gcmc_pme_destroy(p);
+        return pme_fft(r, "PME grid kernel");
+    }
     *out = p;
     return 0;
 }
@@ -287,8 +288,8 @@ int gcmc_pme_create(gcmc_pme **out, const int *grid, const float *box, float bet
 int gcmc_pme_clear(gcmc_pme *p, void *stream)
 {
     if (!p) return gcmc_internal::set_error(GCMC_E_ARG, "null PME plan");
-    hipLaunchKernelGGL(k_pme_fill, dim3((unsigned)((p->ngrid + 255) / 256)), dim3(256), 0, (hipStream_t)stream, p->Q,
-                       p->ngrid);
+    hipLaunchKernelGGL(k_pme_zero64, dim3((unsigned)((p->ngrid + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       p->Qi, p->ngrid);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : pme_hip(e, "PME clear");
 }
@@ -297,8 +298,8 @@ int gcmc_pme_spread(gcmc_pme *p, int n, const float *xyzq, void *stream)
 {
     if (!p || n < 0 || (n > 0 && !xyzq)) return gcmc_internal::set_error(GCMC_E_ARG, "bad PME spread arguments");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_pme_spread, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
-                       (const float4 *)xyzq, p->geo, p->Q);
+    hipLaunchKernelGGL(k_pme_spread, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, (const int *)nullptr,
+                       (const float4 *)xyzq, p->geo, p->Qi);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : pme_hip(e, "PME spread");
 }
@@ -307,6 +308,8 @@ int gcmc_pme_solve(gcmc_pme *p, double *energy, void *stream)
 {
     if (!p) return gcmc_internal::set_error(GCMC_E_ARG, "null PME plan");
     hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_pme_tofloat, dim3((unsigned)((p->ngrid + 255) / 256)), dim3(256), 0, s,
+                       (const unsigned long long *)p->Qi, p->Q, p->ngrid);
     hipfftResult r = hipfftSetStream(p->fwd, s);
     if (r == HIPFFT_SUCCESS) r = hipfftSetStream(p->inv, s);
     if (r == HIPFFT_SUCCESS) r = hipfftExecR2C(p->fwd, p->Q, p->X);
@@ -333,3 +336,23 @@ int gcmc_pme_gather(gcmc_pme *p, int n, const float *xyz, float *phi, void *stre
 }
 
 }  // extern "C"
+
+namespace gcmc_internal {
+// for the engine's Ewald mode: potential grid, grid shape, spread of a
+// device-counted list, counter reset
+const float *pme_phi(const gcmc_pme *p) { return p->Phi; }
+const float *pme_kernel(const gcmc_pme *p) { return p->Gk; }
+void pme_grid(const gcmc_pme *p, int K[3])
+{
+    for (int i = 0; i < 3; ++i) K[i] = p->geo.K[i];
+}
+int pme_spread_counted(gcmc_pme *p, int cap, const int *count, const float4 *xyzq, int *reset, hipStream_t s)
+{
+    if (cap > 0)
+        hipLaunchKernelGGL(gcmc_pme_k::k_pme_spread, dim3((cap + 255) / 256), dim3(256), 0, s, cap, count, xyzq, p->geo,
+                           p->Qi);
+    if (reset) hipLaunchKernelGGL(gcmc_pme_k::k_pme_zero_count, dim3(1), dim3(64), 0, s, reset);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : pme_hip(e, "PME spread");
+}
+}  // namespace gcmc_internal

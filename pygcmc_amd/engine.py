@@ -72,6 +72,8 @@ def lib():
         L.gcmc_engine_last_run_ms.argtypes = [P, P]
         L.gcmc_engine_last_kernel_ms.argtypes = [P, P]
         L.gcmc_engine_launch_info.argtypes = [P, P]
+        L.gcmc_engine_set_ewald.argtypes = [P, I, ctypes.c_float, ctypes.c_float]
+        L.gcmc_engine_ewald_potential.argtypes = [P, I, I, P, P]
         L.gcmc_engine_counters.argtypes = [P, I, P]
         L.gcmc_engine_phase_ticks.argtypes = [P, I, P]
         L.gcmc_engine_sub_ticks.argtypes = [P, I, P]
@@ -166,6 +168,18 @@ class Engine:
                                                 _p(ff), _p(moves), n))
         self.systems.append(sysd)
         return r
+
+    def set_ewald(self, r, beta=0.34, grid_spacing=1.0):
+        """Ewald electrostatics for replica r (before upload): real-space
+        erfc pair terms and PME reciprocal potential; plain mode only."""
+        _check(lib().gcmc_engine_set_ewald(self._h, int(r), float(beta), float(grid_spacing)))
+
+    def ewald_potential(self, r, xyz):
+        """Reciprocal potential (kJ/mol/e) of replica r's current state at points [n, 3]."""
+        xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        phi = np.zeros(len(xyz), np.float32)
+        _check(lib().gcmc_engine_ewald_potential(self._h, int(r), len(xyz), _p(xyz), _p(phi)))
+        return phi
 
     def upload(self):
         _check(lib().gcmc_engine_upload(self._h))
