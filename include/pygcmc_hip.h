@@ -154,6 +154,9 @@ typedef struct gcmc_trace {
     int *idx;           /* [S][K]    */
     int *gpick;         /* [S][K]    */
     int *gacc;          /* [S][K]    */
+    unsigned long long *tick; /* [S][4] device ticks (100 MHz) of the batched form:
+                                 move published, accept start, accept end, next
+                                 move prepared; 0 in the plain form */
 } gcmc_trace;
 
 int gcmc_engine_set_trace(gcmc_engine *e, int replica, int max_steps, int max_k);

@@ -503,13 +503,13 @@ int compute_layout(gcmc_engine *E)
     return 0;
 }
 
-template <int NT, bool S>
+template <int NT, bool S, bool EW>
 void launch_plain_step(gcmc_engine *E, int r)
 {
     const size_t lds = E->lay.total;
-    hipLaunchKernelGGL((k_eval_plain<NT, S>), dim3(2 * E->reps[r]->maxk), dim3(NT), lds, E->stream, E->d_rep,
+    hipLaunchKernelGGL((k_eval_plain<NT, S, EW>), dim3(2 * E->reps[r]->maxk), dim3(NT), lds, E->stream, E->d_rep,
                        E->d_state, r, E->lay);
-    hipLaunchKernelGGL((k_accept_plain<NT, S>), dim3(1), dim3(NT), lds, E->stream, E->d_rep, E->d_state, r,
+    hipLaunchKernelGGL((k_accept_plain<NT, S, EW>), dim3(1), dim3(NT), lds, E->stream, E->d_rep, E->d_state, r,
                        E->lay);
 }
 
@@ -551,7 +551,8 @@ void launch_plain(gcmc_engine *E, int nsteps)
         const int n = std::min(nsteps, std::max(rem, 0));
         HostRep &H = *E->reps[r];
         for (int s = 0; s < n; ++s) {
-            launch_plain_step<NT, S>(E, r);
+            if (H.pme) launch_plain_step<NT, S, true>(E, r);
+            else launch_plain_step<NT, S, false>(E, r);
             if (H.pme) {
                 // charges moved by the step enter the PME grid; new potential
                 gcmc_internal::pme_spread_counted(H.pme, H.rep.pme_cap, H.rep.pme_ndelta, H.rep.pme_delta,
@@ -583,8 +584,12 @@ struct RunPoses {
     static void run(gcmc_engine *E, int npose, int r, int species, const float *dc, const float *dp, const int *dx,
                     double *de, int *dn, uint32_t *dh)
     {
-        hipLaunchKernelGGL((k_eval_poses<NT, S>), dim3(npose), dim3(NT), E->lay.total, E->stream, E->d_rep, r,
-                           species, dc, dp, dx, de, dn, dh, E->lay);
+        if (E->reps[r]->pme)
+            hipLaunchKernelGGL((k_eval_poses<NT, S, true>), dim3(npose), dim3(NT), E->lay.total, E->stream, E->d_rep,
+                               r, species, dc, dp, dx, de, dn, dh, E->lay);
+        else
+            hipLaunchKernelGGL((k_eval_poses<NT, S, false>), dim3(npose), dim3(NT), E->lay.total, E->stream,
+                               E->d_rep, r, species, dc, dp, dx, de, dn, dh, E->lay);
     }
 };
 
@@ -988,6 +993,8 @@ int gcmc_engine_set_trace(gcmc_engine *E, int r, int max_steps, int max_k)
         HIPX(B.alloc(&T.idx, S * K));
         HIPX(B.alloc(&T.gpick, S * K));
         HIPX(B.alloc(&T.gacc, S * K));
+        HIPX(B.alloc(&T.tick, S * 4));
+        HIPX(hipMemset(T.tick, 0, S * 4 * sizeof(unsigned long long)));
         HIPX(hipMemset(T.k, 0, S * sizeof(int)));
         HIPX(hipMemset(T.ngroups, 0, S * sizeof(int)));
         HIPX(hipMemset(T.naccept, 0, S * sizeof(int)));
@@ -1028,6 +1035,7 @@ int gcmc_engine_get_trace(gcmc_engine *E, int r, gcmc_trace *out)
     HIPX(hipMemcpy(out->idx, T.idx, S * K * sizeof(int), hipMemcpyDeviceToHost));
     HIPX(hipMemcpy(out->gpick, T.gpick, S * K * sizeof(int), hipMemcpyDeviceToHost));
     HIPX(hipMemcpy(out->gacc, T.gacc, S * K * sizeof(int), hipMemcpyDeviceToHost));
+    if (out->tick) HIPX(hipMemcpy(out->tick, T.tick, S * 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -147,14 +147,13 @@ __device__ inline float two_diff(float a, float b, float &e)
 // ---- pair energies of the buffered included groups ------------------------
 // lane <-> environment atom (float4 load, coalesced within a group), loop
 // over the fragment atoms held in LDS; FP32 terms, TwoSum into (acc, cmp).
-template <int NT, bool STAGED>
+template <int NT, bool STAGED, bool EW>
 __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
 {
     const int A = wg_scan<NT>(L.incn, n, L.misc);
     const float2 *PT = STAGED ? L.pair : C.pair;
     const float Lx = C.L[0], Ly = C.L[1], Lz = C.L[2];
     const float ix = C.invL[0], iy = C.invL[1], iz = C.invL[2];
-    const bool ew = C.ew != 0;
     const float eb = C.ew_beta;
     for (int t = threadIdx.x; t < A; t += NT) {
         const int g = bsearch_le(L.incn, n, t);
@@ -183,7 +182,7 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
             const float s6 = s2 * s2 * s2;
             const float lj = pp.y * fmaf(s6, s6, -s6);     // 4 eps (s^12 - s^6)
             float el = (fr.w * at.w) * ir;                  // K q_f q_a / r
-            if (ew) el *= erfcf(eb * (r2 * ir));            // Ewald real-space part
+            if (EW) el *= erfcf(eb * (r2 * ir));            // Ewald real-space part
             const float term = lj + el;
             const float s = acc + term;                     // TwoSum
             const float bp = s - acc;
@@ -197,11 +196,12 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
 }
 
 // the energy pass of eval_energy
+template <bool EW>
 struct EnergyPass {
     template <int NT, bool STAGED>
     __device__ int run(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
     {
-        return pair_pass<NT, STAGED>(C, L, n, nf, acc, cmp);
+        return pair_pass<NT, STAGED, EW>(C, L, n, nf, acc, cmp);
     }
 };
 
@@ -359,13 +359,13 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     __syncthreads();
 }
 
-template <int NT, bool STAGED>
+template <int NT, bool STAGED, bool EW = false>
 __device__ __attribute__((always_inline)) inline void eval_energy(const Rep &C, float cx, float cy, float cz, int nf,
                                                                   int exclude, const Lds &L, double *outE, int *outN,
                                                                   uint32_t *outH, int *outA,
                                                                   unsigned long long *ticks = nullptr)
 {
-    EnergyPass P;
+    EnergyPass<EW> P;
     eval_energy_pass<NT, STAGED>(C, cx, cy, cz, nf, exclude, L, P, outE, outN, outH, outA, ticks);
 }
 
@@ -440,7 +440,7 @@ __device__ inline double ewald_recip(const Rep &C, const Species &F, const Lds &
 // ---- one trial evaluation ----------------------------------------------------
 // item < k: trial item (insert/delete) or E_old (translate/rotate);
 // item >= k: E_new of trial item-k (translate/rotate).
-template <int NT, bool STAGED, bool LOCAL>
+template <int NT, bool STAGED, bool LOCAL, bool EW>
 __device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -539,8 +539,8 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
     uint32_t hh;
     unsigned long long ticks[4];
     ticks[0] = wall_clock64() - tq0;
-    eval_energy<NT, STAGED>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat, ticks);
-    if (C.ew) {
+    eval_energy<NT, STAGED, EW>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat, ticks);
+    if (EW) {
         // the moving molecule (delete, translate, rotate) is in the PME grid
         const double er = ewald_recip<NT>(C, F, L, nf, kind == K_INS ? -1 : F.atomBase + slot * nf);
         if (tid == 0) E += er;
@@ -749,7 +749,7 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
 }
 
 // ---- accept phase: decisions and commit ------------------------------------------
-template <int NT>
+template <int NT, bool EW>
 __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
 {
     const int tid = threadIdx.x;
@@ -793,7 +793,7 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
             const int newN = N - 1;
             const int hole_slot = C.tslot[0];
             const int hole = F.startRes + hole_slot, last = F.startRes + newN;
-            if (C.ew) {
+            if (EW) {
                 // the deleted molecule's charges leave the PME grid (the moved
                 // last molecule keeps its coordinates)
                 for (int a = tid; a < nf; a += NT) {
@@ -841,16 +841,16 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
                 C.atype[base + a] = F.ttype[a];
             } else {
                 dst = C.atom[base + a];
-                if (C.ew) C.pme_delta[2 * idx] = make_float4(dst.x, dst.y, dst.z, -F.tmpl[a].w);
+                if (EW) C.pme_delta[2 * idx] = make_float4(dst.x, dst.y, dst.z, -F.tmpl[a].w);
                 dst.x = p.x; dst.y = p.y; dst.z = p.z;
             }
-            if (C.ew) {
+            if (EW) {
                 if (kind == K_INS) C.pme_delta[2 * idx] = make_float4(0.f, 0.f, 0.f, 0.f);
                 C.pme_delta[2 * idx + 1] = make_float4(p.x, p.y, p.z, F.tmpl[a].w);
             }
             C.atom[base + a] = dst;
         }
-        if (C.ew && tid == 0) *C.pme_ndelta = 2 * na * nf;
+        if (EW && tid == 0) *C.pme_ndelta = 2 * na * nf;
         if (tid == 0) {
             for (int q = 0; q < na; ++q) {
                 const int2 ts = L.accl[q];
@@ -950,12 +950,13 @@ __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
 }
 
 // XCD-local ordering.  Every workgroup touching a replica in the batched form
-// runs on the same XCD, whose L2 is shared: a release waits until this
-// thread's memory operations reached L2; an acquire drops the CU's L1 so
-// later loads read L2.  (Agent-scope fences would write back and invalidate
-// the whole L2 of the XCD on every claim.)
-__device__ inline void xcd_release() { __builtin_amdgcn_s_waitcnt(0); }
-__device__ inline void xcd_acquire() { asm volatile("buffer_inv sc0" ::: "memory"); }
+// runs on the same XCD, whose L2 is shared and whose L1s are write-through: a
+// release waits until this thread's memory operations reached L2; an acquire
+// invalidates the CU's L1 at device scope (sc1) so later loads read L2 -- a
+// workgroup-scope invalidate (sc0) leaves L1 as it is, since one workgroup's
+// waves share that L1.  No L2 write-back: replica data never leaves the XCD.
+__device__ inline void xcd_release() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ inline void xcd_acquire() { asm volatile("buffer_inv sc1" ::: "memory"); }
 
 // advance past empty moves and publish the next move with work (batched form)
 template <int NT>
@@ -966,8 +967,7 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
             xcd_release();
             __syncthreads();
             if (threadIdx.x == 0) {
-                __hip_atomic_store(&S.claim, claim_word(S.step, 0, 0), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&S.claim, claim_word(S.step, 0, 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_fetch_add(&sch->finished, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             break;
@@ -976,16 +976,16 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
         if (S.nitems > 0) {
             if (threadIdx.x == 0) {
                 S.t_pub = wall_clock64();
-                __hip_atomic_store(&S.done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_store(&S.done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             xcd_release();
             __syncthreads();
             if (threadIdx.x == 0)
                 __hip_atomic_store(&S.claim, claim_word(S.step, S.nitems, 0), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                                   __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
-        accept_step<NT>(C, S, L);
+        accept_step<NT, false>(C, S, L);
     }
     __syncthreads();
 }
@@ -1020,24 +1020,24 @@ __global__ __launch_bounds__(NT) void k_start(const Rep *reps, State *st, Sched 
     else if (st[r].step < st[r].end) prep_step<NT>(reps[r], st[r], L);
 }
 
-template <int NT, bool STAGED>
+template <int NT, bool STAGED, bool EW>
 __global__ __launch_bounds__(NT) void k_eval_plain(const Rep *reps, const State *st, int r, LdsLayout lay)
 {
     extern __shared__ __align__(16) char smem[];
     const State &S = st[r];
     if (S.step >= S.end || (int)blockIdx.x >= S.nitems) return;
     const Lds L = carve(smem, lay);
-    eval_item<NT, STAGED, false>(reps[r], S, blockIdx.x, L);
+    eval_item<NT, STAGED, false, EW>(reps[r], S, blockIdx.x, L);
 }
 
-template <int NT, bool STAGED>
+template <int NT, bool STAGED, bool EW>
 __global__ __launch_bounds__(NT) void k_accept_plain(const Rep *reps, State *st, int r, LdsLayout lay)
 {
     extern __shared__ __align__(16) char smem[];
     State &S = st[r];
     if (S.step >= S.end) return;
     const Lds L = carve(smem, lay);
-    accept_step<NT>(reps[r], S, L);
+    accept_step<NT, EW>(reps[r], S, L);
     if (S.step < S.end) prep_step<NT>(reps[r], S, L);
 }
 
@@ -1074,9 +1074,11 @@ __global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int 
     while (true) {
         if (tid < 64) {
             // wave 0 polls every replica of this XCD at once, one claim word
-            // per lane.  The polls are loads that bypass L1 (sc0) and are served
-            // by the XCD's L2 in parallel; read-modify-writes of one address
-            // serialize in L2, so an atomic is issued only to claim.
+            // per lane.  Claim words are device-scope: a workgroup-scope load
+            // may hit this CU's L1, which only a claiming workgroup invalidates,
+            // so idle workgroups would keep seeing an old word.  Polls are
+            // loads; an atomic is issued only to claim (read-modify-writes of
+            // one address serialize).
             const int lane = tid;
             int found = -1, item = 0;
             int ctl[3] = {0, 0, 0};
@@ -1098,7 +1100,7 @@ __global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int 
                     unsigned long long w = 0ull;
                     if (b0 + lane < nc) {
                         r = xm.cls_reps[c0 + (rot + b0 + lane) % nc];
-                        w = __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        w = __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     bool avail = r >= 0 && (w & 0xFFFFFull) < ((w >> 20) & 0xFFFFFull);
                     // the replica furthest behind (lowest step) first: the run
@@ -1116,7 +1118,7 @@ __global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int 
                         unsigned long long old = 0ull;
                         if (lane == l)
                             old = __hip_atomic_fetch_add(&st[rr].claim, 1ull, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+                                                         __HIP_MEMORY_SCOPE_AGENT);
                         old = __shfl(old, l, 64);
                         const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
                         if (c < n) {
@@ -1151,26 +1153,35 @@ __global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int 
         const Rep &C = reps[found];
         State &S = st[found];
         const unsigned long long te0 = wall_clock64();
-        eval_item<NT, STAGED, true>(C, S, item, L);
+        eval_item<NT, STAGED, true, false>(C, S, item, L);
         if (tid == 0) count_add<true>(&S.t_eval, wall_clock64() - te0);
         xcd_release();
         __syncthreads();
         if (tid == 0) {
-            const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             L.misc[M_LAST] = (old == S.nitems - 1) ? 1 : 0;
         }
         __syncthreads();
         if (L.misc[M_LAST]) {
             xcd_acquire();
             const unsigned long long ta0 = wall_clock64();
-            if (tid == 0) S.t_step += ta0 - S.t_pub;
-            accept_step<NT>(C, S, L);
+            const unsigned long long tpub = S.t_pub;
+            const int step0 = S.step;
+            if (tid == 0) S.t_step += ta0 - tpub;
+            accept_step<NT, false>(C, S, L);
             const unsigned long long ta1 = wall_clock64();
             advance_and_publish<NT>(C, S, sch, L);
             if (tid == 0) {
                 const unsigned long long ta2 = wall_clock64();
                 S.t_accept += ta1 - ta0;
                 S.t_prep += ta2 - ta1;
+                int row;
+                if (tracing(C, step0, row)) {
+                    C.tr.tick[4 * row + 0] = tpub;
+                    C.tr.tick[4 * row + 1] = ta0;
+                    C.tr.tick[4 * row + 2] = ta1;
+                    C.tr.tick[4 * row + 3] = ta2;
+                }
                 __hip_atomic_fetch_add(&sch->progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
         }
@@ -1185,7 +1196,7 @@ __global__ void k_xcc_probe(int *out)
 }
 
 // energies of given poses (tests and the adjoint's forward)
-template <int NT, bool STAGED>
+template <int NT, bool STAGED, bool EW>
 __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int spi, const float *centres,
                                                   const float *poses, const int *excl, double *E, int *ninc,
                                                   uint32_t *hash, LdsLayout lay)
@@ -1207,9 +1218,9 @@ __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int s
     double e;
     int n, na;
     uint32_t h;
-    eval_energy<NT, STAGED>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n,
-                            &h, &na);
-    if (C.ew) {
+    eval_energy<NT, STAGED, EW>(C, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2], nf, excl[i], L, &e, &n,
+                                &h, &na);
+    if (EW) {
         const int x = excl[i];
         const bool own = x >= F.startRes && x < F.startRes + F.maxNum;
         const double er = ewald_recip<NT>(C, F, L, nf, own ? F.atomBase + (x - F.startRes) * nf : -1);

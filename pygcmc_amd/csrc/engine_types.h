@@ -50,6 +50,7 @@ struct DevTrace {
     int *ninc;
     uint32_t *inchash, *posehash;
     int *idx, *gpick, *gacc;
+    unsigned long long *tick;  // [S][4] publish, accept start, accept end, prep end (batched form)
 };
 
 struct Rep {

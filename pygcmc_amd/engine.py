@@ -39,7 +39,7 @@ class Tuning(ctypes.Structure):
 class CTrace(ctypes.Structure):
     _fields_ = [('max_steps', ctypes.c_int), ('max_k', ctypes.c_int)] + \
         [(n, ctypes.c_void_p) for n in ('k', 'ngroups', 'naccept', 'n_after', 'etot', 'centre', 'e',
-                                        'ninc', 'inchash', 'posehash', 'idx', 'gpick', 'gacc')]
+                                        'ninc', 'inchash', 'posehash', 'idx', 'gpick', 'gacc', 'tick')]
 
 
 TRACE_FIELDS = {
@@ -47,7 +47,7 @@ TRACE_FIELDS = {
     'n_after': (np.int32, ()), 'etot': (np.float64, ()),
     'centre': (np.float32, ('K', 3)), 'e': (np.float64, ('K', 2)), 'ninc': (np.int32, ('K', 2)),
     'inchash': (np.uint32, ('K', 2)), 'posehash': (np.uint32, ('K',)), 'idx': (np.int32, ('K',)),
-    'gpick': (np.int32, ('K',)), 'gacc': (np.int32, ('K',)),
+    'gpick': (np.int32, ('K',)), 'gacc': (np.int32, ('K',)), 'tick': (np.uint64, (4,)),
 }
 
 
@@ -57,6 +57,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f'{LIB_PATH} is missing: build it with `python -m pygcmc_amd.build`')
+        # torch (when installed) brings its own HIP runtime; loading it first
+        # makes this library bind to that same runtime, so tensors and the
+        # engine can share a process in either import order
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         P, I = ctypes.c_void_p, ctypes.c_int
         L.gcmc_last_error.restype = ctypes.c_char_p
