@@ -88,6 +88,8 @@ typedef struct gcmc_tuning {
     int bucket_cap;      /* mobile groups per cell before overflow; 0 = auto */
     int pair_lds;        /* 1 = stage the LJ pair table in LDS when it fits */
     int wg_size;         /* threads per workgroup: 256, 512 or 1024; 0 = 256 */
+    int fp32_coulomb;    /* 1 = Coulomb pair terms in FP32 like the LJ terms; 0 (default) =
+                            Coulomb in FP64 from the exactly formed FP32 displacement */
 } gcmc_tuning;
 
 #define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */

@@ -31,7 +31,7 @@ constexpr double SIGMA_PER_RMINHALF_D = 0.17817974362806785;  // 2^(5/6) / 10
 constexpr float SIGMA_PER_RMINHALF = (float)SIGMA_PER_RMINHALF_D;
 
 struct AdjLds {
-    float2 *tab;    // [X*Y] (100 sigma^2, 4 epsilon) from the per-type parameters
+    float4 *tab;    // [X*Y] (100 sigma^2 hi, 4 epsilon, 100 sigma^2 lo, 0) from the per-type parameters
     float *dq;      // [Y]
     float *inv2e;   // [Y] 1 / (2 |eps|), 0 where eps = 0
     float *gq, *ge, *gr;  // [Y] per-type derivative sums of this pose
@@ -41,14 +41,14 @@ struct AdjLds {
 __host__ __device__ inline int adj_lds_bytes(int X, int Y)
 {
     auto up = [](int x) { return (x + 15) & ~15; };
-    return up(8 * X * Y) + 5 * up(4 * Y) + up(4 * 6 * MAXF);
+    return up(16 * X * Y) + 5 * up(4 * Y) + up(4 * 6 * MAXF);
 }
 
 __device__ inline AdjLds adj_carve(char *b, int X, int Y)
 {
     auto up = [](int x) { return (x + 15) & ~15; };
     AdjLds A;
-    A.tab = (float2 *)b; b += up(8 * X * Y);
+    A.tab = (float4 *)b; b += up(16 * X * Y);
     A.dq = (float *)b; b += up(4 * Y);
     A.inv2e = (float *)b; b += up(4 * Y);
     A.gq = (float *)b; b += up(4 * Y);
@@ -65,7 +65,7 @@ struct AdjPass {
 
     template <int NT, bool STAGED>
     __device__ __attribute__((always_inline)) int run(const Rep &C, const Lds &L, int n, int nf, float &acc,
-                                                      float &cmp)
+                                                      float &cmp, double &acc64)
     {
         const int At = wg_scan<NT>(L.incn, n, L.misc);
         const float Lx = C.L[0], Ly = C.L[1], Lz = C.L[2];
@@ -81,7 +81,7 @@ struct AdjPass {
             for (int f = 0; f < NF; ++f) {
                 if (f < nf) {
                     const float4 fr = L.frag[f];
-                    const float2 pp = A.tab[L.frow[f] + ty];
+                    const float4 pp = A.tab[L.frow[f] + ty];
                     float ex, ey, ez;
                     float dx = two_diff(at.x, fr.x, ex);
                     float dy = two_diff(at.y, fr.y, ey);
@@ -95,7 +95,9 @@ struct AdjPass {
                     float ir = __builtin_amdgcn_rsqf(r2);
                     ir = ir * fmaf(-0.5f * r2 * ir, ir, 1.5f);
                     const float ir2 = ir * ir;
-                    const float s2 = pp.x * ir2;
+                    const float rr = __builtin_amdgcn_rcpf(r2);
+                    float s2 = pp.x * rr;
+                    s2 = fmaf(fmaf(-s2, r2, pp.x) + pp.z, rr, s2);
                     const float s6 = s2 * s2 * s2;
                     const float s12 = s6 * s6;
                     const float lj = pp.y * (s12 - s6);
@@ -163,7 +165,9 @@ __global__ __launch_bounds__(NT) void k_adjoint(const Rep *reps, int r, int spi,
         const int x = q / Y, y = q - x * Y;
         const double sg = 0.5 * ((double)rmin_half[x] + (double)rmin_half[y]) * (double)SIGMA_PER_RMINHALF_D;
         const double ee = 4.184 * 4.184 * fabs((double)eps_kcal[x]) * fabs((double)eps_kcal[y]);
-        P.A.tab[q] = make_float2((float)(100.0 * sg * sg), (float)(4.0 * sqrt(ee)));
+        const double s2 = 100.0 * sg * sg;
+        const float hi = (float)s2;
+        P.A.tab[q] = make_float4(hi, (float)(4.0 * sqrt(ee)), (float)(s2 - (double)hi), 0.0f);
     }
     if (tid < nf) {
         const float *p = poses + ((size_t)i * nf + tid) * 3;

@@ -266,6 +266,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     R.nres = I.totalResNum;
     R.natom = I.totalAtomNum;
     R.nmoves = (int)H.moves.size();
+    R.coul64 = E->tune.fp32_coulomb ? 0 : 1;
     R.seed = I.seed;
     R.maxk = H.maxk;
 
@@ -321,10 +322,14 @@ int upload_rep(gcmc_engine *E, HostRep &H)
         atom[a] = make_float4(A.position[0], A.position[1], A.position[2], A.charge);
         atype[a] = A.type;
     }
-    std::vector<float2> pair((size_t)R.X * R.Y);
+    std::vector<float4> pair((size_t)R.X * R.Y);
     for (size_t i = 0; i < pair.size(); ++i) {
-        double sg = H.ff[2 * i], ep = H.ff[2 * i + 1];
-        pair[i] = make_float2((float)(sg * sg * 100.0), (float)(4.0 * ep));
+        // 100 sigma^2 as an unevaluated FP32 sum hi + lo (exact to ~2^-48);
+        // 4 eps is exact in FP32
+        const double sg = H.ff[2 * i], ep = H.ff[2 * i + 1];
+        const double s2 = sg * sg * 100.0;
+        const float hi = (float)s2;
+        pair[i] = make_float4(hi, (float)(4.0 * ep), (float)(s2 - (double)hi), 0.0f);
     }
     std::vector<Species> sp(T);
     for (int s = 0; s < T; ++s) {
@@ -403,7 +408,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     HIPX(put(&R.res_rng, B, res_rng.data(), res_rng.size()));
     HIPX(put(&R.atom, B, atom.data(), atom.size()));
     HIPX(put(&R.atype, B, atype.data(), atype.size()));
-    HIPX(put((float2 **)&R.pair, B, pair.data(), pair.size()));
+    HIPX(put((float4 **)&R.pair, B, pair.data(), pair.size()));
     HIPX(put((Species **)&R.sp, B, sp.data(), sp.size()));
     HIPX(put((int **)&R.fix_start, B, fstart.data(), fstart.size()));
     HIPX(put((float4 **)&R.fix_rec, B, frec.data(), frec.size()));
@@ -465,7 +470,7 @@ int compute_layout(gcmc_engine *E)
         nst = std::max(nst, H->rep.nst);
         xy = std::max(xy, H->rep.X * H->rep.Y);
     }
-    const int pair_bytes = 8 * xy;
+    const int pair_bytes = 16 * xy;
     E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
     auto up = [](int x) { return (x + 15) & ~15; };
     LdsLayout &l = E->lay;

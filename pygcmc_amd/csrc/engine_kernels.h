@@ -40,7 +40,7 @@ struct Lds {
     int *scell, *sfix, *soff, *incg, *incs, *incn;
     float4 *frag;
     int *frow;
-    float2 *pair;
+    float4 *pair;
     int4 *spt;
     int *misc;
     double *dred;
@@ -62,7 +62,7 @@ __device__ inline Lds carve(char *b, const LdsLayout &l)
     s.incn = (int *)(b + l.incn);
     s.frag = (float4 *)(b + l.frag);
     s.frow = (int *)(b + l.frow);
-    s.pair = (float2 *)(b + (l.pair >= 0 ? l.pair : 0));
+    s.pair = (float4 *)(b + (l.pair >= 0 ? l.pair : 0));
     s.spt = (int4 *)(b + l.spt);
     s.misc = (int *)(b + l.misc);
     s.dred = (double *)(b + l.dred);
@@ -148,13 +148,15 @@ __device__ inline float two_diff(float a, float b, float &e)
 // lane <-> environment atom (float4 load, coalesced within a group), loop
 // over the fragment atoms held in LDS; FP32 terms, TwoSum into (acc, cmp).
 template <int NT, bool STAGED, bool EW>
-__device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
+__device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
 {
     const int A = wg_scan<NT>(L.incn, n, L.misc);
-    const float2 *PT = STAGED ? L.pair : C.pair;
+    const float4 *PT = STAGED ? L.pair : C.pair;
     const float Lx = C.L[0], Ly = C.L[1], Lz = C.L[2];
     const float ix = C.invL[0], iy = C.invL[1], iz = C.invL[2];
     const float eb = C.ew_beta;
+    const bool c64 = !EW && C.coul64 != 0;
+    double accd = 0.0;
     for (int t = threadIdx.x; t < A; t += NT) {
         const int g = bsearch_le(L.incn, n, t);
         const int a = L.incs[g] + (t - L.incn[g]);
@@ -162,7 +164,7 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
         const int ty = C.atype[a];
         for (int f = 0; f < nf; ++f) {
             const float4 fr = L.frag[f];
-            const float2 pp = PT[L.frow[f] + ty];
+            const float4 pp = PT[L.frow[f] + ty];
             // displacement as an exact sum s + e (TwoSum); the image shift
             // s - n L is exact for |n| <= 1 (Sterbenz)
             float ex, ey, ez;
@@ -177,12 +179,45 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
             const float r2 = fmaf(2.0f, r2c, r2h);
             float ir = __builtin_amdgcn_rsqf(r2);
             ir = ir * fmaf(-0.5f * r2 * ir, ir, 1.5f);     // one Newton step
-            const float ir2 = ir * ir;
-            const float s2 = pp.x * ir2;
+            // s^2 = 100 sigma^2 / r^2 to ~0.5 ulp: 100 sigma^2 held exactly as
+            // pp.x + pp.z, quotient with one residual correction.  s^12
+            // multiplies the relative error of s^2 by six, and a rounded
+            // table value would err alike on every pair of a type pair.
+            const float rr = __builtin_amdgcn_rcpf(r2);
+            float s2 = pp.x * rr;
+            s2 = fmaf(fmaf(-s2, r2, pp.x) + pp.z, rr, s2);
             const float s6 = s2 * s2 * s2;
-            const float lj = pp.y * fmaf(s6, s6, -s6);     // 4 eps (s^12 - s^6)
-            float el = (fr.w * at.w) * ir;                  // K q_f q_a / r
-            if (EW) el *= erfcf(eb * (r2 * ir));            // Ewald real-space part
+            float lj = pp.y * fmaf(s6, s6, -s6);           // 4 eps (s^12 - s^6)
+            float el = 0.0f;
+            if (s2 > 1.0f || c64) {
+                // FP64 from the exact displacement (s + e per axis); FP64 FMA
+                // runs at the FP32 rate on this GPU
+                const double ux = (double)dx + (double)ex, uy = (double)dy + (double)ey,
+                             uz = (double)dz + (double)ez;
+                const double r2d = fma(uz, uz, fma(uy, uy, ux * ux));
+                if (s2 > 1.0f) {
+                    // inside sigma s^12 dominates and carries six times the
+                    // relative error of r^2: this rare pair's LJ term in FP64
+                    double q = __builtin_amdgcn_rcp(r2d);
+                    q = q * fma(-r2d, q, 2.0);
+                    q = q * fma(-r2d, q, 2.0);
+                    const double s2d = ((double)pp.x + (double)pp.z) * q;
+                    const double s6d = s2d * s2d * s2d;
+                    accd = fma((double)pp.y, fma(s6d, s6d, -s6d), accd);
+                    lj = 0.0f;
+                }
+                if (c64) {
+                    // K q_f q_a / r
+                    double y = __builtin_amdgcn_rsq(r2d);
+                    y = y * fma(-0.5 * r2d * y, y, 1.5);
+                    y = y * fma(-0.5 * r2d * y, y, 1.5);
+                    accd = fma((double)fr.w * (double)at.w, y, accd);
+                }
+            }
+            if (!c64) {
+                el = (fr.w * at.w) * ir;                    // K q_f q_a / r
+                if (EW) el *= erfcf(eb * (r2 * ir));        // Ewald real-space part
+            }
             const float term = lj + el;
             const float s = acc + term;                     // TwoSum
             const float bp = s - acc;
@@ -192,6 +227,7 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
         }
     }
     __syncthreads();
+    acc64 += accd;
     return A;
 }
 
@@ -199,9 +235,9 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
 template <bool EW>
 struct EnergyPass {
     template <int NT, bool STAGED>
-    __device__ int run(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp)
+    __device__ int run(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
     {
-        return pair_pass<NT, STAGED, EW>(C, L, n, nf, acc, cmp);
+        return pair_pass<NT, STAGED, EW>(C, L, n, nf, acc, cmp, acc64);
     }
 };
 
@@ -260,6 +296,7 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     const int M = wg_scan<NT>(L.soff, ne, L.misc);
     const unsigned long long tk1 = wall_clock64();
     float acc = 0.0f, cmp = 0.0f;
+    double acc64 = 0.0;  // FP64 pair terms (Coulomb)
     int nin = 0, natom = 0;
     uint32_t hs = 0;
     if (tid == 0) L.misc[M_NINC] = 0;
@@ -317,13 +354,13 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
         const int ninc = L.misc[M_NINC];
         if (ninc > L.inc_cap - NT || base + NT >= M) {
             const unsigned long long tp0 = wall_clock64();
-            if (ninc > 0) natom += P.template run<NT, STAGED>(C, L, ninc, nf, acc, cmp);
+            if (ninc > 0) natom += P.template run<NT, STAGED>(C, L, ninc, nf, acc, cmp, acc64);
             tpair += wall_clock64() - tp0;
             if (tid == 0) L.misc[M_NINC] = 0;
             __syncthreads();
         }
     }
-    double v = (double)acc + (double)cmp;
+    double v = (double)acc + (double)cmp + acc64;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         v += __shfl_down(v, off, 64);
@@ -1059,7 +1096,7 @@ __device__ inline int my_xcc() { return (int)(__builtin_amdgcn_s_getreg((20) | (
 // replica finished, or when no replica advanced during spin_limit polls
 // (then sch->abort is set).
 template <int NT, bool STAGED>
-__global__ __launch_bounds__(NT) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
+__global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
                                                LdsLayout lay, int spin_limit, XcdMap xm)
 {
     extern __shared__ __align__(16) char smem[];

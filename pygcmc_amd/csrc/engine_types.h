@@ -62,6 +62,7 @@ struct Rep {
     int G, T, X, Y;
     int nfix, nres, natom, maxk;
     int cap, ovf_base, nst, nmoves;
+    int coul64, pad1;        // Coulomb pair terms in FP64
     uint32_t seed;
     int pad0;
     const float *grid;
@@ -70,7 +71,7 @@ struct Rep {
     int2 *res_rng;
     float4 *atom;
     int *atype;
-    const float2 *pair;
+    const float4 *pair;      // (100 sigma^2 hi, 4 eps, 100 sigma^2 lo, 0) per (fragment type, type)
     const Species *sp;
     const int *fix_start;
     const float4 *fix_rec;   // (x, y, z, gid bits)

@@ -33,7 +33,8 @@ _lib = None
 
 class Tuning(ctypes.Structure):
     _fields_ = [('cell_size', ctypes.c_float), ('persistent_wgs', ctypes.c_int),
-                ('bucket_cap', ctypes.c_int), ('pair_lds', ctypes.c_int), ('wg_size', ctypes.c_int)]
+                ('bucket_cap', ctypes.c_int), ('pair_lds', ctypes.c_int), ('wg_size', ctypes.c_int),
+                ('fp32_coulomb', ctypes.c_int)]
 
 
 class CTrace(ctypes.Structure):
@@ -136,9 +137,9 @@ def host_stencil_covers(centres, groups, cryst, startxyz, cutoff, cell_size=0.0)
 class Engine:
     """Independent GCMC systems on one GPU (see module docstring)."""
 
-    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=1, wg_size=0):
+    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=1, wg_size=0, fp32_coulomb=0):
         self._h = ctypes.c_void_p()
-        t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds, wg_size)
+        t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds, wg_size, fp32_coulomb)
         _check(lib().gcmc_engine_create(ctypes.byref(self._h), ctypes.byref(t)))
         self.systems = []
         self._traces = {}

@@ -95,7 +95,7 @@ def _compare(gt, ot, gpu_sys, ref):
     eg, eo = gt['e'][kmask], ot['e'][kmask]
     err = np.abs(eg - eo)
     print('trial energies (kJ/mol):', band_report(eo, err))
-    assert (err <= e_tol(eo)).all(), (err.max(), eo[err.argmax()])
+    assert (err <= e_tol(eo)).all(), (err.max(), eo.ravel()[err.argmax()])
     gmask = np.arange(gt['gpick'].shape[1])[None, :] < gt['ngroups'][:, None]
     ng = int(gmask.sum())
     mism = int((gt['gpick'][gmask] != ot['gpick'][gmask]).sum() +
@@ -195,3 +195,14 @@ def test_rungcmc_dropin_matches_oracle():
     assert np.array_equal(g['frag']['totalNum'], ref['frag']['totalNum'])
     assert np.array_equal(g['atoms'].view(np.uint8), ref['atoms'].view(np.uint8))
     assert np.array_equal(g['res'].view(np.uint8), ref['res'].view(np.uint8))
+
+
+def test_lockstep_bench_replica():
+    """First replica of the 64-system benchmark (uneven sizes; 8 fragments +
+    water, full confBias): lock-step against the FP64 oracle at the bars."""
+    from pygcmc_amd import synth
+    sizes = synth.replica_sizes(64)
+    s = synth.replica_system(0, int(sizes[0]), mcsteps=300)
+    gt, ot, g, r, st = _lockstep(s, 300, 'batched')
+    err, ng, mism = _compare(gt, ot, g, r)
+    print(f'bench replica 0 ({int(sizes[0])} fixed atoms): max |dE| {err:.3e}, {ng} groups, {mism} mismatches')
