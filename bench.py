@@ -107,6 +107,26 @@ def parity_sample(system, nsteps):
     return float(err[band].max()) if band.any() else 0.0, agree, int(gm.sum()), rel_hi
 
 
+def measured_traffic(steps, replicas, tuning):
+    """HBM bytes of the timed k_batched launch from the committed rocprofv3
+    --pmc record (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known
+    size), when that record was measured on this same workload; else None."""
+    path = os.path.join(ROOT, 'profiles', 'k_batched_64replicas.pmc.txt')
+    if not os.path.exists(path):
+        return None, None
+    rec = {}
+    for line in open(path):
+        if '=' in line and not line.startswith('#'):
+            k, v = line.strip().split('=', 1)
+            rec[k] = v
+    same = (int(rec.get('steps', -1)) == steps and int(rec.get('replicas', -1)) == replicas and
+            float(rec.get('cell_size', -1)) == float(tuning.get('cell_size', 0)) and
+            int(rec.get('wg_size', -1)) == int(tuning.get('wg_size', 0) or 256))
+    if not same:
+        return None, None
+    return float(rec['traffic_bytes']), 'profiles/k_batched_64replicas.pmc.txt (rocprofv3 --pmc, same workload)'
+
+
 def adjoint_bench(npose, reps=5):
     """Forward + backward of the pose-energy adjoint: npose poses (equal
     shares of the 8 fragment types, uniform centres, Shoemake orientations)
@@ -184,7 +204,17 @@ def main():
                          'adjoint or CPU legs')
     ap.add_argument('--adjoint-poses', type=int, default=4096)
     ap.add_argument('--autotune', type=int, default=1, help='pick cell/workgroup sizes by timing short runs')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                    help='only run 3 plain copies of MIB MiB (profiler counter calibration) and exit')
     args = ap.parse_args()
+    if args.copy_probe:
+        import ctypes
+        from pygcmc_amd import engine
+        L = engine.lib()
+        L.gcmc_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int]
+        engine._check(L.gcmc_copy_probe(args.copy_probe << 20, 3))
+        print(json.dumps({'copy_probe_bytes': args.copy_probe << 20, 'copies': 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported in this build '
               '(independent systems would shard one group per GPU, see DESIGN.md)', file=sys.stderr)
@@ -262,6 +292,7 @@ def main():
     cpu_rate, cpu_done, cpu_dt = cpu_baseline(copies(states_w[:4]), args.warmup, args.cpu_seconds)
     log(f'cpu: {cpu_done} steps in {cpu_dt:.1f}s')
 
+    traffic, traffic_src = measured_traffic(args.steps, R, tuning)
     adj = adjoint_bench(args.adjoint_poses) if args.adjoint_poses > 0 else None
     if adj:
         log(f"adjoint: {adj['poses']} poses fwd+bwd in {adj['ms']:.2f} ms")
@@ -301,7 +332,10 @@ def main():
         'adjoint': adj,
         'roofline': {'bound': 'mfma', 'pipe': 'fp32-valu', 'kernel': 'k_batched',
                      'achieved': round(tflops, 3), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': round(tflops / FP32_PEAK_TFLOPS, 4), 'traffic': None,
+                     'frac': round(tflops / FP32_PEAK_TFLOPS, 4), 'traffic': traffic,
+                     'traffic_source': traffic_src,
+                     'hbm_GBps': None if traffic is None else round(traffic / (ms_kernel / 1e3) / 1e9, 1),
+                     'hbm_frac': None if traffic is None else round(traffic / (ms_kernel / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
                      'algorithmic_GBps': round(gbs, 1), 'kernel_ms': round(ms_kernel, 2),
                      'pairs': int(pairs), 'evals': int(evals)},
         'cpu_baseline': {'value': round(cpu_rate, 2), 'unit': 'MC steps/s', 'cores': 1, 'kind': 'port',

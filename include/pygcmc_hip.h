@@ -244,6 +244,11 @@ int gcmc_pme_spread(gcmc_pme *p, int n, const float *xyzq, void *stream);
 int gcmc_pme_solve(gcmc_pme *p, double *energy, void *stream);
 int gcmc_pme_gather(gcmc_pme *p, int n, const float *xyz, float *phi, void *stream);
 
+/* Counter calibration (no counterpart in the reference): `copies` plain
+ * device-to-device copies of `bytes` bytes (a multiple of 16), so a profiler's
+ * FETCH_SIZE / WRITE_SIZE readings can be scaled to bytes moved. */
+int gcmc_copy_probe(long long bytes, int copies);
+
 /* Host execution of the engine's FP32 group-inclusion test -- the code the
  * GPU runs, compiled for the CPU -- for n pairs of points: out[i] = 1 when
  * distanceP(a_i, b_i, cryst) <= cutoff (gcmc/cpp/gcmc_energy.h:115-122). */

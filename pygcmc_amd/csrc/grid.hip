@@ -194,3 +194,34 @@ int gcmc_cavity_grid(int natom, const double *xyz, const double *startxyz, doubl
 }
 
 }  // extern "C"
+
+// ---- counter calibration ------------------------------------------------------
+// A plain device-to-device copy of a known size, for scaling the FETCH_SIZE /
+// WRITE_SIZE counters of a profiler run (bytes read and written are exactly
+// `bytes` per copy).
+namespace gcmc_grid {
+__global__ void k_copy_probe(const float4 *a, float4 *b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+}  // namespace gcmc_grid
+
+extern "C" int gcmc_copy_probe(long long bytes, int copies)
+{
+    using namespace gcmc_grid;
+    if (bytes < 16 || bytes % 16 != 0 || copies < 1) return gcmc_internal::set_error(GCMC_E_ARG, "bad copy size");
+    float4 *a = nullptr, *b = nullptr;
+    hipError_t e = hipMalloc(&a, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemset(a, 1, (size_t)bytes);
+    const size_t n = (size_t)bytes / 16;
+    for (int c = 0; c < copies && e == hipSuccess; ++c) {
+        hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, 0, (const float4 *)a, b, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return e == hipSuccess ? 0 : gcmc_internal::set_error(GCMC_E_HIP, std::string("copy probe: ") + hipGetErrorString(e));
+}
