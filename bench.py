@@ -63,16 +63,17 @@ def counters_sum(eng, R):
     return tot
 
 
-def cpu_baseline(states, first, budget_s):
-    """The FP64 oracle (single thread, reference algorithm: all groups scanned
-    per trial) continuing the same chains from step `first`."""
+def cpu_baseline(states, first, budget_s, nthreads=1):
+    """The FP64 oracle (reference algorithm: all groups scanned per trial;
+    nthreads OpenMP threads over the trials of a move) continuing the same
+    chains from step `first`."""
     from oracle import oracle as O
     done = 0
     t0 = time.perf_counter()
     for s in states:
         step = first
         while time.perf_counter() - t0 < budget_s and step < len(s['moves']):
-            O.run(s, first=step, nsteps=1, nthreads=1)
+            O.run(s, first=step, nsteps=1, nthreads=nthreads)
             step += 1
             done += 1
         if time.perf_counter() - t0 >= budget_s:
@@ -291,6 +292,9 @@ def main():
     # CPU baseline: the oracle continuing the warm chains
     cpu_rate, cpu_done, cpu_dt = cpu_baseline(copies(states_w[:4]), args.warmup, args.cpu_seconds)
     log(f'cpu: {cpu_done} steps in {cpu_dt:.1f}s')
+    ncpu = min(os.cpu_count() or 1, 16)
+    cpu_rate_n, cpu_done_n, cpu_dt_n = cpu_baseline(copies(states_w[:4]), args.warmup, args.cpu_seconds / 3, ncpu)
+    log(f'cpu ({ncpu} threads): {cpu_done_n} steps in {cpu_dt_n:.1f}s')
 
     traffic, traffic_src = measured_traffic(args.steps, R, tuning)
     adj = adjoint_bench(args.adjoint_poses) if args.adjoint_poses > 0 else None
@@ -341,6 +345,9 @@ def main():
         'cpu_baseline': {'value': round(cpu_rate, 2), 'unit': 'MC steps/s', 'cores': 1, 'kind': 'port',
                          'sample': f'FP64 oracle, {cpu_done} moves of systems 0-3 continuing from '
                                    f'move {args.warmup} ({cpu_dt:.1f} s)'},
+        'cpu_baseline_threads': {'value': round(cpu_rate_n, 2), 'unit': 'MC steps/s', 'cores': ncpu,
+                                 'kind': 'port', 'sample': f'same, OpenMP over the trials of a move, '
+                                                           f'{cpu_done_n} moves ({cpu_dt_n:.1f} s)'},
     }
     print(json.dumps(out), flush=True)
 

@@ -442,15 +442,24 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
     const orc_info *info = S->info;
     const float *L = info->cryst;
     float rc = info->cutoff;
-    static __thread float centres[ORC_MAXK][3];
-    static __thread float poses[ORC_MAXK][ORC_MAXF][3];
-    static __thread double e0[ORC_MAXK], e1[ORC_MAXK];
-    static __thread int ninc0[ORC_MAXK], ninc1[ORC_MAXK], slots[ORC_MAXK];
-    static __thread uint32_t ih0[ORC_MAXK], ih1[ORC_MAXK];
-    static __thread int members[ORC_MAXK];
-    static __thread double w[ORC_MAXK];
-    static __thread unsigned char unused[ORC_MAXK];
-    (void)nthreads;
+    /* per-call trial buffers, shared by the OpenMP workers of the trial loop
+       (each trial writes only its own entries) */
+    typedef struct {
+        float centres[ORC_MAXK][3];
+        float poses[ORC_MAXK][ORC_MAXF][3];
+        double e0[ORC_MAXK], e1[ORC_MAXK], w[ORC_MAXK], dtmp[ORC_MAXK];
+        int ninc0[ORC_MAXK], ninc1[ORC_MAXK], slots[ORC_MAXK], members[ORC_MAXK];
+        uint32_t ih0[ORC_MAXK], ih1[ORC_MAXK];
+        unsigned char unused[ORC_MAXK];
+    } orc_buf;
+    orc_buf *bf = (orc_buf *)malloc(sizeof(orc_buf));
+    if (!bf) return -2;
+    float (*centres)[3] = bf->centres;
+    float (*poses)[ORC_MAXF][3] = bf->poses;
+    double *e0 = bf->e0, *e1 = bf->e1, *w = bf->w;
+    int *ninc0 = bf->ninc0, *ninc1 = bf->ninc1, *slots = bf->slots, *members = bf->members;
+    uint32_t *ih0 = bf->ih0, *ih1 = bf->ih1;
+    unsigned char *unused = bf->unused;
 
     for (int si = 0; si < nsteps; ++si) {
         int step = first + si;
@@ -463,7 +472,10 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
         if (kind == 0) k = (N >= F->maxNum) ? 0 : F->confBias;
         else if (kind == 1) k = (N == 0) ? 0 : 1;
         else k = F->confBias < N ? F->confBias : N;
-        if (k > ORC_MAXK) return -1;
+        if (k > ORC_MAXK) {
+            free(bf);
+            return -1;
+        }
         if (st) st->attempts[sp][kind] += 1;
 
         /* picks for delete / translate / rotate */
@@ -547,7 +559,7 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
             }
         } else if (k > 0) {
             double *dE = e0; /* energies used for weights */
-            static __thread double dtmp[ORC_MAXK];
+            double *dtmp = bf->dtmp;
             if (kind != 0) {
                 for (int t = 0; t < k; ++t) dtmp[t] = e1[t] - e0[t];
                 dE = dtmp;
@@ -655,6 +667,7 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
             out->etot[si] = st ? st->e_tot : 0.0;
         }
     }
+    free(bf);
     return 0;
 }
 

@@ -314,3 +314,14 @@ def test_adjoint_epsilon_euler_identity():
             e, gx, gq, ge, gr = O.adjoint(s, spi, c[i], p[i], eps, rmin)
             e_el = O.adjoint(s, spi, c[i], p[i], np.zeros_like(eps), rmin)[0]
             assert abs(np.dot(eps, ge) - (e - e_el)) <= 1e-9 * max(1.0, abs(e - e_el))
+
+
+def test_oracle_threads_do_not_change_the_chain():
+    """Trials of a move are evaluated in parallel; the chain must not depend
+    on the thread count."""
+    s = systems.multi(nsteps=200, n_fixed=2000, box=40.0, seed=31)
+    a, b = O.copy_system(s), O.copy_system(s)
+    O.run(a, nthreads=1)
+    O.run(b, nthreads=4)
+    assert np.array_equal(a['atoms'].view(np.uint8), b['atoms'].view(np.uint8))
+    assert np.array_equal(a['frag']['totalNum'], b['frag']['totalNum'])
