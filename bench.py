@@ -122,7 +122,8 @@ def measured_traffic(steps, replicas, tuning):
             rec[k] = v
     same = (int(rec.get('steps', -1)) == steps and int(rec.get('replicas', -1)) == replicas and
             float(rec.get('cell_size', -1)) == float(tuning.get('cell_size', 0)) and
-            int(rec.get('wg_size', -1)) == int(tuning.get('wg_size', 0) or 256))
+            int(rec.get('wg_size', -1)) == int(tuning.get('wg_size', 0) or 256) and
+            int(rec.get('pair_lds', 1)) == int(tuning.get('pair_lds', 1)))
     if not same:
         return None, None
     return float(rec['traffic_bytes']), 'profiles/k_batched_64replicas.pmc.txt (rocprofv3 --pmc, same workload)'
@@ -200,6 +201,7 @@ def main():
     ap.add_argument('--cell-size', type=float, default=0.0)
     ap.add_argument('--wgs', type=int, default=0)
     ap.add_argument('--wg-size', type=int, default=0)
+    ap.add_argument('--pair-lds', type=int, default=1)
     ap.add_argument('--kernel-only', action='store_true',
                     help='only the timed batched run (for profiler runs): no graph, plain, parity, '
                          'adjoint or CPU legs')
@@ -226,7 +228,7 @@ def main():
     systems, sizes = build_replicas(args.replicas, total)
     R = len(systems)
     log(f'built {R} systems ({int(sizes.sum())} fixed atoms) in {time.time() - t0:.1f}s')
-    tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs, wg_size=args.wg_size)
+    tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs, wg_size=args.wg_size, pair_lds=args.pair_lds)
     tune_table = None
     if args.autotune:
         from pygcmc_amd.autotune import autotune

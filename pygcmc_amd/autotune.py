@@ -1,8 +1,8 @@
 """Autotuner of the batched engine's launch and data-layout parameters.
 
-The engine exposes four tuning knobs (``gcmc_tuning`` in
-``include/pygcmc_hip.h``): cell-list edge, threads per workgroup, number of
-persistent workgroups and the mobile-bucket capacity.  ``autotune`` times a
+The engine exposes tuning knobs (``gcmc_tuning`` in ``include/pygcmc_hip.h``):
+cell-list edge, threads per workgroup, number of persistent workgroups,
+mobile-bucket capacity and whether the pair table is staged in LDS.  ``autotune`` times a
 short batched run of the caller's systems for each candidate setting -- on
 copies of the arrays, so the caller's state is untouched -- and returns the
 fastest setting; ``Engine(**best)`` then runs the real workload with it.
@@ -26,9 +26,10 @@ import numpy as np
 from .engine import Engine
 
 DEFAULT_SPACE = {
-    'cell_size': (4.0, 5.0, 6.0, 7.5),
+    'cell_size': (5.0, 6.0, 7.5, 9.0),
     'wg_size': (256, 512),
     'persistent_wgs': (0,),
+    'pair_lds': (0, 1),
 }
 
 

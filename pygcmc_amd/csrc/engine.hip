@@ -497,6 +497,17 @@ int compute_layout(gcmc_engine *E)
     l.w = a; a = up(a + 8 * E->maxk);
     l.val = a; a = up(a + 8 * E->maxk);
     l.accl = a; a = up(a + 8 * E->maxk);
+    l.rsel = a; a = up(a + 8 * E->maxk);
+    l.racc = a; a = up(a + 8 * E->maxk);
+    l.ea = a; a = up(a + 8 * E->maxk);
+    // adjacency rows of a move staged in LDS for its accept phase, when they fit
+    const int adj_bytes = 4 * E->maxk * ((E->maxk + 31) / 32);
+    if (a + adj_bytes + 4096 <= 49152) {
+        l.adjl = a;
+        a = up(a + adj_bytes);
+    } else {
+        l.adjl = -1;
+    }
     l.J = 0;  // prep runs after accept in the same workgroup
     o = std::max(eval_end, a);
     l.spt = o; o = up(o + 16 * MAXT);

@@ -47,6 +47,8 @@ struct Lds {
     int *members;
     double *w, *val;
     int2 *accl;
+    uint32_t *adjl;   // adjacency rows staged for the accept phase (null: read C.adj)
+    double *rsel, *racc, *ea;  // accept: selection / acceptance draw per group, exp term per trial
     int *J;
     int inc_cap;
 };
@@ -70,6 +72,10 @@ __device__ inline Lds carve(char *b, const LdsLayout &l)
     s.w = (double *)(b + l.w);
     s.val = (double *)(b + l.val);
     s.accl = (int2 *)(b + l.accl);
+    s.adjl = l.adjl >= 0 ? (uint32_t *)(b + l.adjl) : nullptr;
+    s.rsel = (double *)(b + l.rsel);
+    s.racc = (double *)(b + l.racc);
+    s.ea = (double *)(b + l.ea);
     s.J = (int *)(b + l.J);
     s.inc_cap = l.inc_cap;
     return s;
@@ -478,7 +484,7 @@ __device__ inline double ewald_recip(const Rep &C, const Species &F, const Lds &
 // item < k: trial item (insert/delete) or E_old (translate/rotate);
 // item >= k: E_new of trial item-k (translate/rotate).
 template <int NT, bool STAGED, bool LOCAL, bool EW>
-__device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L)
+__device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L, bool stage = true)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long tq0 = wall_clock64();
@@ -490,9 +496,13 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
         t = item - k;
         phase = 1;
     }
-    if (STAGED)
-        for (int i = tid; i < C.X * C.Y; i += NT) L.pair[i] = C.pair[i];
-    load_species_table(C, L);
+    // the pair and species tables stay in LDS while a workgroup keeps
+    // evaluating items of one replica (stage = false)
+    if (stage) {
+        if (STAGED)
+            for (int i = tid; i < C.X * C.Y; i += NT) L.pair[i] = C.pair[i];
+        load_species_table(C, L);
+    }
     const float4 tcv = C.tc[t];
     const int slot = (kind == K_INS) ? -1 : C.tslot[t];
     const int gid = (kind == K_INS) ? -1 : F.startRes + slot;
@@ -666,7 +676,16 @@ __device__ inline void cell_rename(const Rep &C, int from, int to)
 //   insert:  min(1, cav/(m P)/(N+1) exp(B - beta E))   (gcmc_move_add.h:164-174)
 //   move:    min(1, 1/(m P) exp(-beta dE))             (gcmc_move_trn.h:215-222)
 // On accept the unused trials within the cutoff of the pick are dropped.
-// Trial values are staged in L.val.
+// Trial values are staged in L.val, adjacency rows in L.adjl (when the layout
+// has room; else read from C.adj).  The weight sum and the cumulative pick
+// are wave-parallel: lane l sums a contiguous chunk of the members in order,
+// the chunk sums are scanned across lanes, and the lane whose chunk holds
+// the crossing walks it -- one fixed FP64 summation order.
+__device__ inline uint32_t adj_word(const Rep &C, const Lds &L, int nw, int t, int w)
+{
+    return L.adjl ? L.adjl[t * nw + w] : C.adj[t * ADJW + w];
+}
+
 __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int row, bool trace)
 {
     const int lane = threadIdx.x & 63;
@@ -681,7 +700,6 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
     }
     int N = S.totalNum[spi];
     const int maxN = F.maxNum;
-    const double B = F.B;
     const double cav = (double)C.cav;
     int g = 0, na = 0;
     double et = S.energy;
@@ -691,7 +709,7 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
         const int f = __ffsll((long long)bal) - 1;
         const uint32_t wf = __shfl(u, f, 64);
         const int it = f * 32 + __ffs(wf) - 1;
-        uint32_t m = (lane < nw) ? (C.adj[it * ADJW + lane] & u) : 0u;
+        uint32_t m = (lane < nw) ? (adj_word(C, L, nw, it, lane) & u) : 0u;
         if (lane == (it >> 5)) m |= 1u << (it & 31);
         u &= ~m;
         const int cnt = __popc(m);
@@ -710,49 +728,74 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
             L.members[o++] = lane * 32 + b;
         }
         wave_sync();
-        double emin = __builtin_huge_val();
-        for (int q = lane; q < mtot; q += 64) {
-            const double v = L.val[L.members[q]];
-            emin = v < emin ? v : emin;
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double o2 = __shfl_xor(emin, off, 64);
-            emin = o2 < emin ? o2 : emin;
-        }
-        for (int q = lane; q < mtot; q += 64) L.w[q] = exp(-BETA * (L.val[L.members[q]] - emin));
-        wave_sync();
-        double sumP = 0.0;
-        for (int q = 0; q < mtot; ++q) sumP += L.w[q];
-        const u32x4 xr = draw(C.seed, (uint32_t)step, (uint32_t)g, P_DECIDE);
-        const double ran_sel = u01d(xr.x, xr.y);
-        const double ran_acc = u01d(xr.z, xr.w);
+        const double ran_sel = L.rsel[g];
+        const double ran_acc = L.racc[g];
         int pick = it, pq = 0;
-        if (sumP != 0.0) {
-            double cum = 0.0;
-            pick = L.members[mtot - 1];
-            pq = mtot - 1;
-            for (int q = 0; q < mtot; ++q) {
-                cum += L.w[q] / sumP;
-                if (ran_sel < cum) {
-                    pick = L.members[q];
-                    pq = q;
-                    break;
-                }
+        double sumP = 1.0, P = 1.0;
+        if (mtot > 1) {
+            double emin = __builtin_huge_val();
+            for (int q = lane; q < mtot; q += 64) {
+                const double v = L.val[L.members[q]];
+                emin = v < emin ? v : emin;
             }
+    #pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double o2 = __shfl_xor(emin, off, 64);
+                emin = o2 < emin ? o2 : emin;
+            }
+            for (int q = lane; q < mtot; q += 64) L.w[q] = exp(-BETA * (L.val[L.members[q]] - emin));
+            wave_sync();
+            // chunk sums in member order, scanned across lanes
+            const int per = (mtot + 63) >> 6;
+            const int q0 = min(lane * per, mtot), q1 = min(q0 + per, mtot);
+            double part = 0.0;
+            for (int q = q0; q < q1; ++q) part += L.w[q];
+            double incl = part;
+    #pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const double y = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += y;
+            }
+            sumP = __shfl(incl, 63, 64);
+            if (sumP != 0.0) {
+                // first member q with sum_{j<=q} w_j > ran_sel * sumP (the last one
+                // when rounding leaves the target at the total)
+                const double target = ran_sel * sumP;
+                const double excl = incl - part;
+                const unsigned long long hit = __ballot(q1 > q0 && excl <= target && target < incl);
+                int sel = mtot - 1;
+                if (hit) {
+                    const int l = __ffsll((long long)hit) - 1;
+                    int found = q1 - 1;
+                    if (lane == l) {
+                        double c = excl;
+                        for (int q = q0; q < q1; ++q) {
+                            c += L.w[q];
+                            if (target < c) {
+                                found = q;
+                                break;
+                            }
+                        }
+                    }
+                    sel = __shfl(found, l, 64);
+                }
+                pq = sel;
+                pick = L.members[sel];
+            }
+            P = sumP != 0.0 ? L.w[pq] / sumP : 1.0;
         }
-        const double P = sumP != 0.0 ? L.w[pq] / sumP : 1.0;
+        // (a single-member group has weight exp(0) = 1: pick = leader, P = 1)
         const double vp = L.val[pick];
         double pp;
         if (ins)
-            pp = (cav / ((double)mtot * P)) / ((double)N + 1.0) * exp(B - BETA * vp);
+            pp = (cav / ((double)mtot * P)) / ((double)N + 1.0) * L.ea[pick];
         else
-            pp = (1.0 / ((double)mtot * P)) * exp(-BETA * vp);
+            pp = (1.0 / ((double)mtot * P)) * L.ea[pick];
         const double p = pp < 1.0 ? pp : 1.0;
         const bool acc = ran_acc < p;
         int rec = acc ? 1 : 0;
         if (acc) {
-            const uint32_t ap = (lane < nw) ? C.adj[pick * ADJW + lane] : 0u;
+            const uint32_t ap = (lane < nw) ? adj_word(C, L, nw, pick, lane) : 0u;
             u &= ~ap;
             if (ins) {
                 if (N < maxN) {
@@ -863,6 +906,26 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
             }
         }
     } else if (k > 0 && kind != K_DEL) {
+        if (L.adjl) {
+            const int nw = (k + 31) >> 5;
+            for (int q = tid; q < k * nw; q += NT) {
+                const int t = q / nw, w = q - t * nw;
+                L.adjl[q] = C.adj[t * ADJW + w];
+            }
+        }
+        // everything of the serial selection that does not depend on earlier
+        // groups: the variates of group g (at most k groups) and each trial's
+        // exp(B - beta E) (insert) or exp(-beta dE) (move)
+        {
+            const double Bx = (kind == K_INS) ? F.B : 0.0;
+            for (int t = tid; t < k; t += NT) {
+                const u32x4 xr = draw(C.seed, (uint32_t)step, (uint32_t)t, P_DECIDE);
+                L.rsel[t] = u01d(xr.x, xr.y);
+                L.racc[t] = u01d(xr.z, xr.w);
+                L.ea[t] = exp(Bx - BETA * L.val[t]);
+            }
+        }
+        __syncthreads();
         if ((tid >> 6) == 0) greedy_select(C, S, L, row, trace);
         __syncthreads();
         const int na = L.misc[M_ACC];
@@ -1108,6 +1171,7 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
     if (nc <= 0) return;
     const int rot = (int)(blockIdx.x / (unsigned)xm.nclass) % nc;
     int idle = 0, seen = -1;
+    int cached = -1;  // replica whose pair and species tables are in LDS
     while (true) {
         if (tid < 64) {
             // wave 0 polls every replica of this XCD at once, one claim word
@@ -1190,7 +1254,8 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
         const Rep &C = reps[found];
         State &S = st[found];
         const unsigned long long te0 = wall_clock64();
-        eval_item<NT, STAGED, true, false>(C, S, item, L);
+        eval_item<NT, STAGED, true, false>(C, S, item, L, found != cached);
+        cached = found;
         if (tid == 0) count_add<true>(&S.t_eval, wall_clock64() - te0);
         xcd_release();
         __syncthreads();
@@ -1200,6 +1265,9 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
         }
         __syncthreads();
         if (L.misc[M_LAST]) {
+            // accept and prep are this replica's serial critical path: issue
+            // ahead of the evaluation waves sharing the SIMDs
+            __builtin_amdgcn_s_setprio(3);
             xcd_acquire();
             const unsigned long long ta0 = wall_clock64();
             const unsigned long long tpub = S.t_pub;
@@ -1221,6 +1289,8 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
                 }
                 __hip_atomic_fetch_add(&sch->progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+            __builtin_amdgcn_s_setprio(0);
+            cached = -1;  // accept and prep reuse the evaluation region
         }
         __syncthreads();
     }

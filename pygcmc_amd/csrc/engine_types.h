@@ -134,6 +134,8 @@ struct LdsLayout {
     int spt;                       // species table int4[MAXT] (startRes, maxNum, atomBase, nf)
     int misc, dred;                // int[MISC_INTS], double[MAXWAVE]
     int members, w, val, accl;     // accept: int[maxk], double[maxk] x2, int2[maxk]
+    int adjl;                      // accept: adjacency rows uint32[maxk * ceil(maxk/32)] or -1
+    int rsel, racc, ea;            // accept: double[maxk] each -- draws per group, exp terms per trial
     int J;                         // prep: int[maxk]
     int inc_cap;                   // included groups buffered before a pair pass
     int total;
