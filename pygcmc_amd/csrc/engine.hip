@@ -465,18 +465,20 @@ int upload_rep(gcmc_engine *E, HostRep &H)
 
 int compute_layout(gcmc_engine *E)
 {
-    int nst = 1, xy = 1;
+    // staged pair table: the rows of one species' atoms (nf x Y entries)
+    int nst = 1, fy = 1;
     for (HostRep *H : E->reps) {
         nst = std::max(nst, H->rep.nst);
-        xy = std::max(xy, H->rep.X * H->rep.Y);
+        for (const AtomArray &F : H->frag) fy = std::max(fy, F.num_atoms * H->rep.Y);
     }
-    const int pair_bytes = 16 * xy;
+    const int pair_bytes = 16 * fy;
     E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
     auto up = [](int x) { return (x + 15) & ~15; };
     LdsLayout &l = E->lay;
     l.inc_cap = 2 * E->nt;
     int o = 0;
-    l.scell = o; o = up(o + 4 * (nst + 2));
+    l.smob = o; o = up(o + 4 * (nst + 2));
+    l.sfst = o; o = up(o + 4 * (nst + 2));
     l.sfix = o; o = up(o + 4 * (nst + 2));
     l.soff = o; o = up(o + 4 * (nst + 2));
     l.incg = o; o = up(o + 4 * (l.inc_cap + 1));
@@ -764,6 +766,37 @@ int gcmc_engine_upload(gcmc_engine *E)
     for (HostRep *H : E->reps) {
         int rc = upload_rep(E, *H);
         if (rc) return rc;
+    }
+    // Scheduling priority of the batched form: the estimated critical-path
+    // cost of a replica's moves still to run.  A move costs a fixed part
+    // (publish, accept, prep, one evaluation's latency) plus one unit per
+    // evaluation beyond the first that the replica's XCD runs (k trials, 2k
+    // for translate / rotate with k <= the initial count of the species).
+    // Prefix sums per replica, scaled so the largest total fits 24 bits.
+    {
+        std::vector<std::vector<unsigned long long>> cum(E->reps.size());
+        unsigned long long top = 1;
+        for (size_t r = 0; r < E->reps.size(); ++r) {
+            const HostRep &H = *E->reps[r];
+            std::vector<unsigned long long> &c = cum[r];
+            c.assign(H.moves.size() + 1, 0ull);
+            for (size_t s = 0; s < H.moves.size(); ++s) {
+                const int kind = H.moves[s] & 3;
+                const AtomArray &F = H.frag[H.moves[s] >> 2];
+                long long n = kind == K_DEL ? 1 : std::min(F.confBias, H.maxk);
+                if (kind == K_TRN || kind == K_ROT) n = 2 * std::min<long long>(n, std::max(F.totalNum, 1));
+                c[s + 1] = c[s] + 55ull + (unsigned long long)std::max<long long>(n, 1);
+            }
+            top = std::max(top, c.back());
+        }
+        int shift = 0;
+        while ((top >> shift) >= (1ull << 24)) ++shift;
+        for (size_t r = 0; r < E->reps.size(); ++r) {
+            HostRep &H = *E->reps[r];
+            std::vector<uint32_t> w(cum[r].size());
+            for (size_t i = 0; i < w.size(); ++i) w[i] = (uint32_t)(cum[r][i] >> shift);
+            HIPX(put((uint32_t **)&H.rep.wcum, H.buf, w.data(), w.size()));
+        }
     }
     E->nt = (E->tune.wg_size == 512 || E->tune.wg_size == 1024) ? E->tune.wg_size : 256;
     int rc = compute_layout(E);

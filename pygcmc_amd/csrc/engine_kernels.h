@@ -21,6 +21,25 @@
 
 namespace gcmc {
 
+// The buffers a Rep points to are all in device global memory, but a pointer
+// read out of a struct is generic to the compiler, and generic (flat) loads
+// count against the LDS counter as well: a wait for an LDS read then also
+// waits for every outstanding flat load.  gbl() restates the address space so
+// the hot loads are global_load / global_store.  (The host pass only parses
+// device functions; address space 1 means nothing there.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+#else
+template <class T>
+using gptr = T *;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> gbl(T *p)
+{
+    return (gptr<T>)p;
+}
+
 // scratch ints of a workgroup (Lds::misc)
 enum Misc : int {
     M_SCAN = 0,      // [0, NW]   scan partials, total at NW
@@ -37,7 +56,7 @@ enum Misc : int {
 };
 
 struct Lds {
-    int *scell, *sfix, *soff, *incg, *incs, *incn;
+    int *smob, *sfst, *sfix, *soff, *incg, *incs, *incn;
     float4 *frag;
     int *frow;
     float4 *pair;
@@ -56,7 +75,8 @@ struct Lds {
 __device__ inline Lds carve(char *b, const LdsLayout &l)
 {
     Lds s;
-    s.scell = (int *)(b + l.scell);
+    s.smob = (int *)(b + l.smob);
+    s.sfst = (int *)(b + l.sfst);
     s.sfix = (int *)(b + l.sfix);
     s.soff = (int *)(b + l.soff);
     s.incg = (int *)(b + l.incg);
@@ -140,6 +160,26 @@ __device__ inline int bsearch_le(const int *a, int n, int m)
     return lo;
 }
 
+// bsearch_le for two keys at once (independent LDS reads in flight together)
+__device__ inline void bsearch_le2(const int *a, int n, int m0, int m1, int &g0, int &g1)
+{
+    int lo0 = 0, hi0 = n - 1, lo1 = 0, hi1 = n - 1;
+    while (lo0 < hi0 || lo1 < hi1) {
+        const int mid0 = (lo0 + hi0 + 1) >> 1, mid1 = (lo1 + hi1 + 1) >> 1;
+        const int v0 = a[mid0], v1 = a[mid1];
+        if (lo0 < hi0) {
+            if (v0 <= m0) lo0 = mid0;
+            else hi0 = mid0 - 1;
+        }
+        if (lo1 < hi1) {
+            if (v1 <= m1) lo1 = mid1;
+            else hi1 = mid1 - 1;
+        }
+    }
+    g0 = lo0;
+    g1 = lo1;
+}
+
 // a - b = s + e exactly (Knuth TwoSum of a and -b); returns s
 __device__ inline float two_diff(float a, float b, float &e)
 {
@@ -157,20 +197,24 @@ template <int NT, bool STAGED, bool EW>
 __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
 {
     const int A = wg_scan<NT>(L.incn, n, L.misc);
-    const float4 *PT = STAGED ? L.pair : C.pair;
+    const auto atom = gbl(C.atom);
+    const auto atype = gbl(C.atype);
+    const auto GPT = gbl(C.pair);
     const float Lx = C.L[0], Ly = C.L[1], Lz = C.L[2];
     const float ix = C.invL[0], iy = C.invL[1], iz = C.invL[2];
     const float eb = C.ew_beta;
     const bool c64 = !EW && C.coul64 != 0;
     double accd = 0.0;
-    for (int t = threadIdx.x; t < A; t += NT) {
-        const int g = bsearch_le(L.incn, n, t);
-        const int a = L.incs[g] + (t - L.incn[g]);
-        const float4 at = C.atom[a];
-        const int ty = C.atype[a];
+    // one environment atom against every fragment atom; a thread takes atoms
+    // t, t + NT, t + 2 NT, ... in that order (the order of its sums)
+    auto atom_terms = [&](const float4 at, const int ty) {
         for (int f = 0; f < nf; ++f) {
             const float4 fr = L.frag[f];
-            const float4 pp = PT[L.frow[f] + ty];
+            float4 pp;
+            if constexpr (STAGED)
+                pp = L.pair[L.frow[f] + ty];
+            else
+                pp = GPT[L.frow[f] + ty];
             // displacement as an exact sum s + e (TwoSum); the image shift
             // s - n L is exact for |n| <= 1 (Sterbenz)
             float ex, ey, ez;
@@ -231,6 +275,21 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
             acc = s;
             cmp += err;
         }
+    };
+    // two atoms per thread per round, both loads issued before either is used
+    for (int t0 = threadIdx.x; t0 < A; t0 += 2 * NT) {
+        const int t1 = t0 + NT;
+        const bool two = t1 < A;
+        int g0, g1;
+        bsearch_le2(L.incn, n, t0, two ? t1 : t0, g0, g1);
+        const int a0 = L.incs[g0] + (t0 - L.incn[g0]);
+        const int a1 = two ? L.incs[g1] + (t1 - L.incn[g1]) : a0;
+        const float4 at0 = atom[a0];
+        const int ty0 = atype[a0];
+        const float4 at1 = atom[a1];
+        const int ty1 = atype[a1];
+        atom_terms(at0, ty0);
+        if (two) atom_terms(at1, ty1);
     }
     __syncthreads();
     acc64 += accd;
@@ -256,6 +315,20 @@ __device__ inline void load_species_table(const Rep &C, const Lds &L)
     }
 }
 
+// pair-table rows of species spi's atoms into LDS, atom f's row at f * Y
+// (L.frow[f] = f * Y when staged)
+template <int NT>
+__device__ inline void stage_species_rows(const Rep &C, int spi, const Lds &L)
+{
+    const auto &F = gbl(C.sp)[spi];
+    const auto GPT = gbl(C.pair);
+    const int Y = C.Y, n = F.nf * Y;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const int f = i / Y;
+        L.pair[i] = GPT[F.trow[f] + (i - f * Y)];
+    }
+}
+
 // ---- calcEnergy over the cell list -----------------------------------------
 // Candidates: fixed and mobile groups of the stencil cells around the test
 // centre plus the overflow bucket; kept when gid != exclude and the FP32
@@ -270,15 +343,21 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     unsigned long long tk0 = wall_clock64(), tpair = 0;
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const auto stencil = gbl(C.stencil);
+    const auto fix_start = gbl(C.fix_start);
+    const auto mob_cnt = gbl(C.mob_cnt);
+    const auto fix_rec = gbl(C.fix_rec);
+    const auto fix_rng = gbl(C.fix_rng);
+    const auto mob_rec = gbl(C.mob_rec);
     const int cc = cell_index(C.L, C.start, C.inv_h, C.nc, cx, cy, cz);
     const int c2 = cc % C.nc[2];
     const int c1 = (cc / C.nc[2]) % C.nc[1];
     const int c0 = cc / (C.nc[1] * C.nc[2]);
     const int ne = C.nst + 1;
     for (int e = tid; e < ne; e += NT) {
-        int cell, fc, mc;
+        int cell, fs, fc, mc;
         if (e < C.nst) {
-            const int code = C.stencil[e];
+            const int code = stencil[e];
             // stencil offsets span at most one box per axis: wrap by one add
             int i0 = c0 + (code & 255) - 128;
             int i1 = c1 + ((code >> 8) & 255) - 128;
@@ -287,14 +366,17 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
             i1 += i1 < 0 ? C.nc[1] : (i1 >= C.nc[1] ? -C.nc[1] : 0);
             i2 += i2 < 0 ? C.nc[2] : (i2 >= C.nc[2] ? -C.nc[2] : 0);
             cell = (i0 * C.nc[1] + i1) * C.nc[2] + i2;
-            fc = C.fix_start[cell + 1] - C.fix_start[cell];
-            mc = C.mob_cnt[cell];
+            fs = fix_start[cell];
+            fc = fix_start[cell + 1] - fs;
+            mc = mob_cnt[cell];
+            L.smob[e] = cell * C.cap;
         } else {
-            cell = C.ncell;
+            fs = 0;
             fc = 0;
-            mc = C.mob_cnt[C.ncell];
+            mc = mob_cnt[C.ncell];
+            L.smob[e] = C.ovf_base;
         }
-        L.scell[e] = cell;
+        L.sfst[e] = fs;
         L.sfix[e] = fc;
         L.soff[e] = fc + mc;
     }
@@ -315,17 +397,16 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
         if (m < M) {
             const int e = bsearch_le(L.soff, ne, m);
             const int local = m - L.soff[e];
-            const int cell = L.scell[e];
             const int fc = L.sfix[e];
             float4 p;
             if (local < fc) {
-                const int q = C.fix_start[cell] + local;
-                p = C.fix_rec[q];
-                rr = C.fix_rng[q];
+                const int q = L.sfst[e] + local;
+                p = fix_rec[q];
+                rr = fix_rng[q];
                 gid = __float_as_int(p.w);
             } else {
-                const int idx = (cell < C.ncell ? cell * C.cap : C.ovf_base) + (local - fc);
-                p = C.mob_rec[idx];
+                const int idx = L.smob[e] + (local - fc);
+                p = mob_rec[idx];
                 gid = __float_as_int(p.w);
                 for (int s = 0; s < C.T; ++s) {
                     const int4 sp = L.spt[s];
@@ -439,6 +520,23 @@ __device__ inline void count_add(T *p, T v)
     else __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// statistics of one evaluation (counters of the replica's State)
+struct ItemStats {
+    unsigned long long ticks[4];
+    unsigned long long nat, nin, nf;
+};
+
+template <bool LOCAL>
+__device__ inline void add_item_stats(State &W, const ItemStats &s)
+{
+    for (int i = 0; i < 4; ++i) count_add<LOCAL>(&W.t_sub[i], s.ticks[i]);
+    count_add<LOCAL>(&W.n_eval, 1ull);
+    count_add<LOCAL>(&W.n_pair, s.nat * s.nf);
+    count_add<LOCAL>(&W.n_atom, s.nat);
+    count_add<LOCAL>(&W.n_group, s.nin);
+    count_add<LOCAL>(&W.n_fatom, s.nf);
+}
+
 // ---- Ewald reciprocal term of a pose ------------------------------------------
 // sum_f q_f phi(x_f) over the pose held in L.frag, phi the PME potential of the
 // current state.  When the molecule itself is in that state (base >= 0: its
@@ -484,29 +582,32 @@ __device__ inline double ewald_recip(const Rep &C, const Species &F, const Lds &
 // item < k: trial item (insert/delete) or E_old (translate/rotate);
 // item >= k: E_new of trial item-k (translate/rotate).
 template <int NT, bool STAGED, bool LOCAL, bool EW>
-__device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L, bool stage = true)
+// Thread 0 adds the statistics to S, or leaves them in *defer (thread 0's
+// copy) for the caller to add once the item's completion is published.
+__device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L, bool stage_rep = true,
+                                 bool stage_rows = true, ItemStats *defer = nullptr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long tq0 = wall_clock64();
     const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
-    const Species &F = C.sp[spi];
+    const auto &F = gbl(C.sp)[spi];
     const int nf = F.nf;
+    const auto tc = gbl(C.tc);
+    const auto atom = gbl(C.atom);
     int t = item, phase = 0;
     if (kind >= K_TRN && item >= k) {
         t = item - k;
         phase = 1;
     }
-    // the pair and species tables stay in LDS while a workgroup keeps
-    // evaluating items of one replica (stage = false)
-    if (stage) {
-        if (STAGED)
-            for (int i = tid; i < C.X * C.Y; i += NT) L.pair[i] = C.pair[i];
-        load_species_table(C, L);
-    }
-    const float4 tcv = C.tc[t];
-    const int slot = (kind == K_INS) ? -1 : C.tslot[t];
+    // the species table stays in LDS while a workgroup keeps evaluating
+    // items of one replica (stage_rep = false), the species' pair rows while
+    // they are also of one species (stage_rows = false)
+    if (stage_rep) load_species_table(C, L);
+    if (STAGED && stage_rows) stage_species_rows<NT>(C, spi, L);
+    const float4 tcv = tc[t];
+    const int slot = (kind == K_INS) ? -1 : gbl(C.tslot)[t];
     const int gid = (kind == K_INS) ? -1 : F.startRes + slot;
-    const float4 cold = (gid >= 0) ? C.res_pos[gid] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 cold = (gid >= 0) ? gbl(C.res_pos)[gid] : make_float4(0.f, 0.f, 0.f, 0.f);
     float cx, cy, cz;
     if (kind == K_INS || phase == 1 || kind == K_DEL) {
         cx = tcv.x; cy = tcv.y; cz = tcv.z;
@@ -514,6 +615,11 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
         cx = cold.x; cy = cold.y; cz = cold.z;
     }
     const bool newpose = (kind == K_INS) || (kind == K_DEL) || (phase == 1);
+    // the first block of trial centres for the adjacency row below, loaded
+    // now so the load overlaps the pose work
+    const bool adjrow = kind != K_DEL && phase == 0;
+    float4 cj0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (adjrow && tid < k) cj0 = tc[tid];
     if (tid < nf) {
         const int a = tid;
         const float4 tm = F.tmpl[a];
@@ -526,7 +632,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
             px = v[0] + cx; py = v[1] + cy; pz = v[2] + cz;
         } else {
             const int base = F.atomBase + slot * nf;
-            const float4 cur = C.atom[base + a];
+            const float4 cur = atom[base + a];
             if (kind == K_DEL || phase == 0) {
                 px = cur.x; py = cur.y; pz = cur.z;
             } else if (kind == K_TRN) {
@@ -545,7 +651,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
                 float cen0 = 0.f, cen1 = 0.f, cen2 = 0.f;
                 const float fn = (float)nf;
                 for (int b = 0; b < nf; ++b) {
-                    const float4 q = C.atom[base + b];
+                    const float4 q = atom[base + b];
                     cen0 = cen0 + q.x / fn;
                     cen1 = cen1 + q.y / fn;
                     cen2 = cen2 + q.z / fn;
@@ -558,25 +664,25 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
             }
         }
         L.frag[a] = make_float4(px, py, pz, F.kq[a]);
-        L.frow[a] = F.trow[a];
-        if (kind == K_INS || phase == 1) C.pose[t * MAXF + a] = make_float4(px, py, pz, 0.f);
+        L.frow[a] = STAGED ? a * C.Y : F.trow[a];
+        if (kind == K_INS || phase == 1) gbl(C.pose)[t * MAXF + a] = make_float4(px, py, pz, 0.f);
     }
     __syncthreads();
 
     // adjacency row of trial t over the trial centres (grouping of the accept phase)
-    if (kind != K_DEL && phase == 0) {
+    if (adjrow) {
         for (int j0 = 0; j0 < k; j0 += NT) {
             const int j = j0 + tid;
             bool b = false;
             if (j < k && j != t) {
-                const float4 cj = C.tc[j];
+                const float4 cj = (j0 == 0) ? cj0 : tc[j];
                 b = within(tcv.x, tcv.y, tcv.z, cj.x, cj.y, cj.z, C.ax, C.r2max);
             }
             const unsigned long long bal = __ballot(b);
             if (lane == 0) {
                 const int wi = (j0 + wave * 64) >> 5;
-                if (wi < ADJW) C.adj[t * ADJW + wi] = (uint32_t)bal;
-                if (wi + 1 < ADJW) C.adj[t * ADJW + wi + 1] = (uint32_t)(bal >> 32);
+                if (wi < ADJW) gbl(C.adj)[t * ADJW + wi] = (uint32_t)bal;
+                if (wi + 1 < ADJW) gbl(C.adj)[t * ADJW + wi + 1] = (uint32_t)(bal >> 32);
             }
         }
     }
@@ -593,14 +699,14 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
         if (tid == 0) E += er;
     }
     if (tid == 0) {
-        C.E[phase * k + t] = E;
-        State &W = const_cast<State &>(S);
-        for (int i = 0; i < 4; ++i) count_add<LOCAL>(&W.t_sub[i], ticks[i]);
-        count_add<LOCAL>(&W.n_eval, 1ull);
-        count_add<LOCAL>(&W.n_pair, (unsigned long long)nat * (unsigned long long)nf);
-        count_add<LOCAL>(&W.n_atom, (unsigned long long)nat);
-        count_add<LOCAL>(&W.n_group, (unsigned long long)nin);
-        count_add<LOCAL>(&W.n_fatom, (unsigned long long)nf);
+        gbl(C.E)[phase * k + t] = E;
+        ItemStats st;
+        for (int i = 0; i < 4; ++i) st.ticks[i] = ticks[i];
+        st.nat = (unsigned long long)nat;
+        st.nin = (unsigned long long)nin;
+        st.nf = (unsigned long long)nf;
+        if (defer) *defer = st;
+        else add_item_stats<LOCAL>(const_cast<State &>(S), st);
         int row;
         if (tracing(C, step, row) && t < C.tr.max_k) {
             const int q = row * C.tr.max_k + t;
@@ -828,6 +934,189 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
     }
 }
 
+// 64-bit wave values by lane index (idx uniform across the wave)
+__device__ inline double lane_d(double v, int idx)
+{
+    const int i = __builtin_amdgcn_readfirstlane(idx);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), i);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), i);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline unsigned long long lane_u64(unsigned long long v, int idx)
+{
+    const int i = __builtin_amdgcn_readfirstlane(idx);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, i);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), i);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ inline int lane_i(int v, int idx)
+{
+    return __builtin_amdgcn_readlane(v, __builtin_amdgcn_readfirstlane(idx));
+}
+// lane r receives v of the lane whose dst is r (dst a permutation of 0..63)
+__device__ inline double permute_d(int dst, double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_permute(dst * 4, (int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_ds_permute(dst * 4, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// greedy_select for k <= 64 with the per-trial values in registers: lane t
+// holds trial t's value, exp term, adjacency row (bit j: trial j within the
+// cutoff) and slot, and group t's two variates.  Group members are compacted
+// into lanes 0..m-1 in trial order (ds_permute), so the weights, their sums
+// and the decisions are those of greedy_select, bit for bit.  Wave 0 only.
+__device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int row, bool trace, double val,
+                                       double ea, double rsel, double racc, unsigned long long adj, int tslot)
+{
+    const int lane = threadIdx.x & 63;
+    const int kind = S.kind, spi = S.sp, k = S.k;
+    const Species &F = C.sp[spi];
+    const bool ins = kind == K_INS;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long U = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
+    int N = S.totalNum[spi];
+    const int maxN = F.maxNum;
+    const double cav = (double)C.cav;
+    int g = 0, na = 0;
+    double et = S.energy;
+    while (U != 0ull) {
+        const int it = __ffsll((long long)U) - 1;
+        const unsigned long long M = (lane_u64(adj, it) & U) | (1ull << it);
+        U &= ~M;
+        const int mtot = __popcll(M);
+        const double ran_sel = lane_d(rsel, g);
+        const double ran_acc = lane_d(racc, g);
+        int pick = it;
+        double P = 1.0;
+        if (mtot > 1) {
+            const bool in = ((M >> lane) & 1ull) != 0ull;
+            double emin = in ? val : __builtin_huge_val();
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double o2 = __shfl_xor(emin, off, 64);
+                emin = o2 < emin ? o2 : emin;
+            }
+            const double wl = in ? exp(-BETA * (val - emin)) : 0.0;
+            const int dst = in ? __popcll(M & below) : mtot + __popcll(~M & below);
+            const double w = permute_d(dst, wl);                      // lane q: member q's weight
+            const int tix = __builtin_amdgcn_ds_permute(dst * 4, lane);  // lane q: member q's trial
+            const double part = lane < mtot ? w : 0.0;
+            double incl = part;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const double y = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += y;
+            }
+            const double sumP = lane_d(incl, 63);
+            int pq = 0;
+            if (sumP != 0.0) {
+                const double target = ran_sel * sumP;
+                const double excl = incl - part;
+                const unsigned long long hit = __ballot(lane < mtot && excl <= target && target < incl);
+                pq = hit ? __ffsll((long long)hit) - 1 : mtot - 1;
+                pick = lane_i(tix, pq);
+                P = lane_d(w, pq) / sumP;
+            }
+        }
+        const double vp = lane_d(val, pick);
+        const double eap = lane_d(ea, pick);
+        double pp;
+        if (ins)
+            pp = (cav / ((double)mtot * P)) / ((double)N + 1.0) * eap;
+        else
+            pp = (1.0 / ((double)mtot * P)) * eap;
+        const double p = pp < 1.0 ? pp : 1.0;
+        const bool acc = ran_acc < p;
+        int rec = acc ? 1 : 0;
+        if (acc) {
+            U &= ~lane_u64(adj, pick);
+            if (ins) {
+                if (N < maxN) {
+                    if (lane == 0) L.accl[na] = make_int2(pick, N);
+                    N += 1;
+                    na += 1;
+                    et += vp;
+                } else {
+                    rec = 2;
+                }
+            } else {
+                const int sl = lane_i(tslot, pick);
+                if (lane == 0) L.accl[na] = make_int2(pick, sl);
+                na += 1;
+                et += vp;
+            }
+        }
+        if (trace && lane == 0 && g < C.tr.max_k) {
+            C.tr.gpick[row * C.tr.max_k + g] = pick;
+            C.tr.gacc[row * C.tr.max_k + g] = rec;
+        }
+        g += 1;
+    }
+    if (lane == 0) {
+        L.misc[M_ACC] = na;
+        L.misc[M_NG] = g;
+        L.misc[M_N] = N;
+        S.energy = et;
+        S.acc[spi * 4 + kind] += na;
+    }
+}
+
+// ---- commit of the accepted trials (L.accl[0..na): trial, slot) ------------------
+// GupdateAdd / GupdateTrn / GupdateRot: copy committed poses, then the group
+// centres and cell-list entries in acceptance order (thread 0).
+template <int NT, bool EW>
+__device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int row, bool trace)
+{
+    const int tid = threadIdx.x;
+    const int kind = S.kind, spi = S.sp;
+    const Species &F = C.sp[spi];
+    const int nf = F.nf;
+    const int na = L.misc[M_ACC];
+    // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses
+    for (int idx = tid; idx < na * nf; idx += NT) {
+        const int q = idx / nf, a = idx - q * nf;
+        const int2 ts = L.accl[q];
+        const int base = F.atomBase + ts.y * nf;
+        const float4 p = C.pose[ts.x * MAXF + a];
+        float4 dst;
+        if (kind == K_INS) {
+            dst = make_float4(p.x, p.y, p.z, F.tmpl[a].w);
+            C.atype[base + a] = F.ttype[a];
+        } else {
+            dst = C.atom[base + a];
+            if (EW) C.pme_delta[2 * idx] = make_float4(dst.x, dst.y, dst.z, -F.tmpl[a].w);
+            dst.x = p.x; dst.y = p.y; dst.z = p.z;
+        }
+        if (EW) {
+            if (kind == K_INS) C.pme_delta[2 * idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+            C.pme_delta[2 * idx + 1] = make_float4(p.x, p.y, p.z, F.tmpl[a].w);
+        }
+        C.atom[base + a] = dst;
+    }
+    if (EW && tid == 0) *C.pme_ndelta = 2 * na * nf;
+    if (tid == 0) {
+        for (int q = 0; q < na; ++q) {
+            const int2 ts = L.accl[q];
+            const int gid = F.startRes + ts.y;
+            const float4 c = C.tc[ts.x];
+            if (kind == K_INS) {
+                C.res_pos[gid] = make_float4(c.x, c.y, c.z, 0.f);
+                cell_insert(C, gid, c.x, c.y, c.z);
+                if (trace && ts.x < C.tr.max_k) C.tr.idx[row * C.tr.max_k + ts.x] = ts.y;
+            } else if (kind == K_TRN) {
+                const float4 old = C.res_pos[gid];
+                C.res_pos[gid] = make_float4(c.x, c.y, c.z, old.w);
+                cell_remove(C, gid);
+                cell_insert(C, gid, c.x, c.y, c.z);
+            }
+        }
+        if (kind == K_INS) S.totalNum[spi] = L.misc[M_N];
+    }
+}
+
 // ---- accept phase: decisions and commit ------------------------------------------
 template <int NT, bool EW>
 __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
@@ -838,8 +1127,11 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
     const int nf = F.nf;
     int row;
     const bool trace = tracing(C, step, row);
-    // stage the trial values (E for insert/delete, E_new - E_old otherwise)
-    for (int t = tid; t < k; t += NT) L.val[t] = (kind == K_TRN || kind == K_ROT) ? C.E[k + t] - C.E[t] : C.E[t];
+    // stage the trial values (E for insert/delete, E_new - E_old otherwise);
+    // with at most 64 trials of an insert or move they stay in registers
+    const bool in_regs = k <= 64 && kind != K_DEL;
+    if (!in_regs)
+        for (int t = tid; t < k; t += NT) L.val[t] = (kind == K_TRN || kind == K_ROT) ? C.E[k + t] - C.E[t] : C.E[t];
     if (tid == 0) {
         S.att[spi * 4 + kind] += 1;
         L.misc[M_ACC] = 0;
@@ -905,6 +1197,29 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
                 L.misc[M_N] = newN;
             }
         }
+    } else if (k > 0 && in_regs) {
+        // lane t of wave 0 holds trial t's values (greedy_select64)
+        if ((tid >> 6) == 0) {
+            const int t = tid;
+            double val = 0.0, ea = 0.0, rsel = 0.0, racc = 0.0;
+            unsigned long long adj = 0ull;
+            int sl = 0;
+            if (t < k) {
+                const auto E = gbl(C.E);
+                val = (kind == K_TRN || kind == K_ROT) ? E[k + t] - E[t] : E[t];
+                const auto A = gbl(C.adj);
+                adj = (unsigned long long)A[t * ADJW];
+                if (k > 32) adj |= (unsigned long long)A[t * ADJW + 1] << 32;
+                if (kind != K_INS) sl = gbl(C.tslot)[t];
+                const u32x4 xr = draw(C.seed, (uint32_t)step, (uint32_t)t, P_DECIDE);
+                rsel = u01d(xr.x, xr.y);
+                racc = u01d(xr.z, xr.w);
+                ea = exp(((kind == K_INS) ? F.B : 0.0) - BETA * val);
+            }
+            greedy_select64(C, S, L, row, trace, val, ea, rsel, racc, adj, sl);
+        }
+        __syncthreads();
+        commit_accepted<NT, EW>(C, S, L, row, trace);
     } else if (k > 0 && kind != K_DEL) {
         if (L.adjl) {
             const int nw = (k + 31) >> 5;
@@ -928,47 +1243,7 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
         __syncthreads();
         if ((tid >> 6) == 0) greedy_select(C, S, L, row, trace);
         __syncthreads();
-        const int na = L.misc[M_ACC];
-        // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses
-        for (int idx = tid; idx < na * nf; idx += NT) {
-            const int q = idx / nf, a = idx - q * nf;
-            const int2 ts = L.accl[q];
-            const int base = F.atomBase + ts.y * nf;
-            const float4 p = C.pose[ts.x * MAXF + a];
-            float4 dst;
-            if (kind == K_INS) {
-                dst = make_float4(p.x, p.y, p.z, F.tmpl[a].w);
-                C.atype[base + a] = F.ttype[a];
-            } else {
-                dst = C.atom[base + a];
-                if (EW) C.pme_delta[2 * idx] = make_float4(dst.x, dst.y, dst.z, -F.tmpl[a].w);
-                dst.x = p.x; dst.y = p.y; dst.z = p.z;
-            }
-            if (EW) {
-                if (kind == K_INS) C.pme_delta[2 * idx] = make_float4(0.f, 0.f, 0.f, 0.f);
-                C.pme_delta[2 * idx + 1] = make_float4(p.x, p.y, p.z, F.tmpl[a].w);
-            }
-            C.atom[base + a] = dst;
-        }
-        if (EW && tid == 0) *C.pme_ndelta = 2 * na * nf;
-        if (tid == 0) {
-            for (int q = 0; q < na; ++q) {
-                const int2 ts = L.accl[q];
-                const int gid = F.startRes + ts.y;
-                const float4 c = C.tc[ts.x];
-                if (kind == K_INS) {
-                    C.res_pos[gid] = make_float4(c.x, c.y, c.z, 0.f);
-                    cell_insert(C, gid, c.x, c.y, c.z);
-                    if (trace && ts.x < C.tr.max_k) C.tr.idx[row * C.tr.max_k + ts.x] = ts.y;
-                } else if (kind == K_TRN) {
-                    const float4 old = C.res_pos[gid];
-                    C.res_pos[gid] = make_float4(c.x, c.y, c.z, old.w);
-                    cell_remove(C, gid);
-                    cell_insert(C, gid, c.x, c.y, c.z);
-                }
-            }
-            if (kind == K_INS) S.totalNum[spi] = L.misc[M_N];
-        }
+        commit_accepted<NT, EW>(C, S, L, row, trace);
     }
     if (tid == 0) {
         if (trace) {
@@ -1058,6 +1333,15 @@ __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
 __device__ inline void xcd_release() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 __device__ inline void xcd_acquire() { asm volatile("buffer_inv sc1" ::: "memory"); }
 
+// claim-word priority of a replica's published move: lower is claimed first,
+// the replica with the most estimated work left first -- the run lasts as long
+// as its slowest chain
+__device__ inline int claim_prio(const Rep &C, const State &S)
+{
+    const uint32_t left = C.wcum[S.end] - C.wcum[S.step];
+    return 0xFFFFFF - (int)(left < 0xFFFFFFu ? left : 0xFFFFFFu);
+}
+
 // advance past empty moves and publish the next move with work (batched form)
 template <int NT>
 __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L)
@@ -1067,7 +1351,7 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
             xcd_release();
             __syncthreads();
             if (threadIdx.x == 0) {
-                __hip_atomic_store(&S.claim, claim_word(S.step, 0, 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&S.claim, claim_word(0xFFFFFF, 0, 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_fetch_add(&sch->finished, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             break;
@@ -1081,7 +1365,7 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
             xcd_release();
             __syncthreads();
             if (threadIdx.x == 0)
-                __hip_atomic_store(&S.claim, claim_word(S.step, S.nitems, 0), __ATOMIC_RELAXED,
+                __hip_atomic_store(&S.claim, claim_word(claim_prio(C, S), S.nitems, 0), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
@@ -1159,7 +1443,7 @@ __device__ inline int my_xcc() { return (int)(__builtin_amdgcn_s_getreg((20) | (
 // replica finished, or when no replica advanced during spin_limit polls
 // (then sch->abort is set).
 template <int NT, bool STAGED>
-__global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
+__global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
                                                LdsLayout lay, int spin_limit, XcdMap xm)
 {
     extern __shared__ __align__(16) char smem[];
@@ -1171,7 +1455,9 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
     if (nc <= 0) return;
     const int rot = (int)(blockIdx.x / (unsigned)xm.nclass) % nc;
     int idle = 0, seen = -1;
-    int cached = -1;  // replica whose pair and species tables are in LDS
+    // replica whose species table, and replica * MAXT + species whose pair
+    // rows, are in LDS
+    int cached_rep = -1, cached_rows = -1;
     while (true) {
         if (tid < 64) {
             // wave 0 polls every replica of this XCD at once, one claim word
@@ -1204,11 +1490,17 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
                         w = __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                     bool avail = r >= 0 && (w & 0xFFFFFull) < ((w >> 20) & 0xFFFFFull);
-                    // the replica furthest behind (lowest step) first: the run
-                    // lasts as long as its slowest chain
-                    const unsigned step = (unsigned)(w >> 40);
+                    // The move with the fewest unclaimed items first: a replica
+                    // waiting on one or a few evaluations is on its own critical
+                    // chain, while a move of hundreds of trials occupies the XCD's
+                    // workgroups for several rounds whichever order they come in.
+                    // Ties (and moves over 255 items) by claim_prio: the most
+                    // estimated work left first.
+                    const unsigned long long unc = ((w >> 20) & 0xFFFFFull) - (w & 0xFFFFFull);
+                    const unsigned size = unc > 255ull ? 255u : (unsigned)unc;
+                    const unsigned prio = (unsigned)(w >> 40) >> 8;
                     while (__ballot(avail) != 0ull) {
-                        unsigned key = avail ? ((step << 6) | (unsigned)lane) : 0xFFFFFFFFu;
+                        unsigned key = avail ? ((size << 22) | (prio << 6) | (unsigned)lane) : 0xFFFFFFFFu;
 #pragma unroll
                         for (int off = 32; off > 0; off >>= 1) {
                             const unsigned o2 = __shfl_xor(key, off, 64);
@@ -1254,14 +1546,21 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
         const Rep &C = reps[found];
         State &S = st[found];
         const unsigned long long te0 = wall_clock64();
-        eval_item<NT, STAGED, true, false>(C, S, item, L, found != cached);
-        cached = found;
-        if (tid == 0) count_add<true>(&S.t_eval, wall_clock64() - te0);
+        const int rows = found * MAXT + S.sp;
+        ItemStats ist;
+        eval_item<NT, STAGED, true, false>(C, S, item, L, found != cached_rep, rows != cached_rows, &ist);
+        cached_rep = found;
+        cached_rows = rows;
+        const unsigned long long te1 = wall_clock64();
         xcd_release();
         __syncthreads();
         if (tid == 0) {
             const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             L.misc[M_LAST] = (old == S.nitems - 1) ? 1 : 0;
+            // statistics after the completion count: the release above does
+            // not wait for them
+            add_item_stats<true>(S, ist);
+            count_add<true>(&S.t_eval, te1 - te0);
         }
         __syncthreads();
         if (L.misc[M_LAST]) {
@@ -1290,7 +1589,7 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_batched(const Rep *reps, Stat
                 __hip_atomic_fetch_add(&sch->progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __builtin_amdgcn_s_setprio(0);
-            cached = -1;  // accept and prep reuse the evaluation region
+            cached_rep = cached_rows = -1;  // accept and prep reuse the evaluation region
         }
         __syncthreads();
     }
@@ -1313,13 +1612,12 @@ __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int s
     const Rep &C = reps[r];
     const Species &F = C.sp[spi];
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x;
-    if (STAGED)
-        for (int q = tid; q < C.X * C.Y; q += NT) L.pair[q] = C.pair[q];
+    if (STAGED) stage_species_rows<NT>(C, spi, L);
     load_species_table(C, L);
     if (tid < nf) {
         const float *p = poses + ((size_t)i * nf + tid) * 3;
         L.frag[tid] = make_float4(p[0], p[1], p[2], F.kq[tid]);
-        L.frow[tid] = F.trow[tid];
+        L.frow[tid] = STAGED ? tid * C.Y : F.trow[tid];
     }
     __syncthreads();
     double e;

@@ -67,6 +67,7 @@ struct Rep {
     int pad0;
     const float *grid;
     const int *moves;
+    const uint32_t *wcum;    // [nmoves+1] estimated critical-path cost of moves [0, s), scaled (scheduling)
     float4 *res_pos;
     int2 *res_rng;
     float4 *atom;
@@ -127,10 +128,10 @@ struct Sched {
 
 // byte offsets of the dynamic LDS carve-up (host computes, device applies)
 struct LdsLayout {
-    int scell, sfix, soff;         // stencil entries (int[nst+2])
+    int smob, sfst, sfix, soff;    // stencil entries (int[nst+2]): mobile-slot base, fixed start, fixed count, offset
     int incg, incs, incn;          // included groups (int[inc_cap+1])
     int frag, frow;                // fragment pose (float4[MAXF], int[MAXF])
-    int pair;                      // float2[X*Y] or -1
+    int pair;                      // float4[nf*Y] (one species' rows) or -1
     int spt;                       // species table int4[MAXT] (startRes, maxNum, atomBase, nf)
     int misc, dred;                // int[MISC_INTS], double[MAXWAVE]
     int members, w, val, accl;     // accept: int[maxk], double[maxk] x2, int2[maxk]
@@ -141,9 +142,11 @@ struct LdsLayout {
     int total;
 };
 
-__host__ __device__ inline unsigned long long claim_word(int epoch, int nitems, int claimed)
+// claim word of a replica: scheduling priority (24 bits), items published
+// (20 bits), items claimed (20 bits)
+__host__ __device__ inline unsigned long long claim_word(int prio, int nitems, int claimed)
 {
-    return ((unsigned long long)(uint32_t)(epoch & 0xFFFFFF) << 40) |
+    return ((unsigned long long)(uint32_t)(prio & 0xFFFFFF) << 40) |
            ((unsigned long long)(uint32_t)(nitems & 0xFFFFF) << 20) |
            (unsigned long long)(uint32_t)(claimed & 0xFFFFF);
 }
