@@ -129,6 +129,38 @@ def measured_traffic(steps, replicas, tuning):
     return float(rec['traffic_bytes']), 'profiles/k_batched_64replicas.pmc.txt (rocprofv3 --pmc, same workload)'
 
 
+def setup_bench():
+    """Set-up of a 4wp7-sized system (30,520 fixed atoms, 57,077 waters,
+    methanols; the size of pygcmc's logged run, test/4wp7/gcmc.75.out:60-105)
+    from PDB + TOP + force-field files to the seven runGCMC arrays: file
+    loading, then set-up with the reference's grid order (host) and with the
+    grid built on the GPU.  pygcmc logged 14.2 s of Python set-up for its
+    run (gcmc.75.out:111)."""
+    import shutil
+    import tempfile
+    from pygcmc_amd import synth
+    from pygcmc_amd.system import GCMCSystem
+    d = tempfile.mkdtemp(prefix='gcmc_setup_')
+    try:
+        base = synth.write_large_files(d)
+        out = {}
+        for order in ('cpython', 'linear'):
+            s = GCMCSystem(fragment_names=['MEOH', 'SOL'], conc=[0.25, 55.0], muex=[-2.04, -5.04], conf=[10, 2],
+                           mctime=[0.2, 0.8], mcsteps=100, seed=1, ff_dir=os.path.join(d, 'charmm36.ff'),
+                           grid_order=order)
+            s.load(base + '.pdb', top=base + '.top')
+            s.setup()
+            out['load_s'] = round(s.timings['load_s'], 3)
+            out['setup_s_' + ('host_grid' if order == 'cpython' else 'gpu_grid')] = round(s.timings['setup_s'], 3)
+            info = s.arrays[0][0]
+            out['pdb_atoms'] = len(s.pdb)
+            out['grid_points'] = int(info['totalGridNum'])
+        out['reference_python_setup_s'] = 14.2
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def adjoint_bench(npose, reps=5):
     """Forward + backward of the pose-energy adjoint: npose poses (equal
     shares of the 8 fragment types, uniform centres, Shoemake orientations)
@@ -209,6 +241,8 @@ def main():
                     help='only the timed batched run (for profiler runs): no graph, plain, parity, '
                          'adjoint or CPU legs')
     ap.add_argument('--adjoint-poses', type=int, default=4096)
+    ap.add_argument('--setup-bench', type=int, default=1,
+                    help='time set-up of a 4wp7-sized system from PDB/TOP/force-field files')
     ap.add_argument('--autotune', type=int, default=0, help='pick cell/workgroup sizes by timing short runs')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only run 3 plain copies of MIB MiB (profiler counter calibration) and exit')
@@ -303,6 +337,9 @@ def main():
 
     traffic, traffic_src = measured_traffic(args.steps, R, tuning)
     adj = adjoint_bench(args.adjoint_poses) if args.adjoint_poses > 0 else None
+    setup = setup_bench() if args.setup_bench else None
+    if setup:
+        log(f"set-up from files: {setup}")
     if adj:
         log(f"adjoint: {adj['poses']} poses fwd+bwd in {adj['ms']:.2f} ms")
 
@@ -339,6 +376,7 @@ def main():
         'tuning': tuning,
         'autotune_ms': None if tune_table is None else [[t, round(ms, 3)] for t, ms in tune_table],
         'adjoint': adj,
+        'setup_from_files': setup,
         'roofline': {'bound': 'mfma', 'pipe': 'fp32-valu', 'kernel': 'k_batched',
                      'achieved': round(tflops, 3), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': round(tflops / FP32_PEAK_TFLOPS, 4), 'traffic': traffic,

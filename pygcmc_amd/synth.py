@@ -409,3 +409,146 @@ def slab(seed=5000, n_fixed=300000, box=(150.0, 150.0, 120.0), thickness=40.0,
     return build_system(seed=seed, fixed_xyz=xyz, fixed_charge=q, fixed_type_names=tnames,
                         protein_lj=lj, species_specs=sp, cryst=cryst, mcsteps=mcsteps,
                         cutoff=cutoff, pme=pme)
+
+
+# ---------------------------------------------------------------------------
+# systems in pyGCMC's input formats (PDB + GROMACS .top / CHARMM .psf + a
+# force-field directory), for the file readers and set-up timing
+# ---------------------------------------------------------------------------
+
+# name, Rmin/2 (A), |eps| (kcal/mol), mass: CHARMM36 / CGenFF values
+FILE_TYPES = [('OT', 1.7682, 0.1521, 15.9994), ('HT', 0.2245, 0.046, 1.008),
+              ('CG331', 2.05, 0.078, 12.011), ('HGA3', 1.34, 0.024, 1.008),
+              ('OG311', 1.765, 0.192, 15.9994), ('HGP1', 0.2245, 0.046, 1.008),
+              ('NH1', 1.85, 0.2, 14.007), ('H', 0.2245, 0.046, 1.008), ('CT1', 2.275, 0.032, 12.011),
+              ('HB1', 1.32, 0.022, 1.008), ('CT3', 2.06, 0.078, 12.011), ('HA3', 1.34, 0.024, 1.008),
+              ('C', 2.0, 0.11, 12.011), ('O', 1.7, 0.12, 15.9994)]
+# an alanine residue: name, type, charge
+FILE_ALA = [('N', 'NH1', -0.47), ('HN', 'H', 0.31), ('CA', 'CT1', 0.07), ('HA', 'HB1', 0.09),
+            ('CB', 'CT3', -0.27), ('HB1', 'HA3', 0.09), ('HB2', 'HA3', 0.09), ('HB3', 'HA3', 0.09),
+            ('C', 'C', 0.51), ('O', 'O', -0.51)]
+# rigid molecules: name, type, charge, xyz (A)
+FILE_MOLECULES = {
+    'MEOH': [('CB', 'CG331', -0.04, (-0.046, 0.664, 0.0)), ('OG', 'OG311', -0.65, (-0.046, -0.758, 0.0)),
+             ('HG1', 'HGP1', 0.42, (-0.996, -1.012, 0.0)), ('HB1', 'HGA3', 0.09, (-1.086, 0.992, 0.0)),
+             ('HB2', 'HGA3', 0.09, (0.437, 1.082, 0.890)), ('HB3', 'HGA3', 0.09, (0.437, 1.082, -0.890))],
+    'SOL': [('OW', 'OT', -0.834, (0.0, 0.0, 0.0)), ('HW1', 'HT', 0.417, (0.9572, 0.0, 0.0)),
+            ('HW2', 'HT', 0.417, (-0.2400, 0.9266, 0.0))],
+}
+
+
+def _pdb_record(serial, name, res, resnr, xyz):
+    nm = (' ' + name) if len(name) < 4 else name
+    return 'ATOM  %5d %-4s %-4s%5d    %8.3f%8.3f%8.3f  1.00  0.00' % (
+        (serial - 1) % 99999 + 1, nm, res, (resnr - 1) % 9999 + 1, xyz[0], xyz[1], xyz[2])
+
+
+def _itp_atoms(atoms, resname, masses):
+    out = ['[ atoms ]', ';   nr  type  resnr residue  atom  cgnr  charge  mass']
+    for i, (name, ty, q) in enumerate(atoms):
+        out.append('%6d %10s %6d %6s %6s %6d %10.3f %10.4f' % (i + 1, ty, 1, resname, name, i + 1, q, masses[ty]))
+    return out
+
+
+def write_forcefield(ffdir, nbfix=(('NH1', 'OT', 0.3, 0.7), ('HGA3', 'CG331', 0.32, 0.12)),
+                     pairtypes=(('CG331', 'HGA3', 0.33, 0.11),)):
+    """A force-field directory as pygcmc reads it: ffnonbonded.itp
+    ([ atomtypes ] in GROMACS units, sigma = Rmin/2 2^(5/6)/10 nm, epsilon =
+    4.184 |eps| kJ/mol, plus [ pairtypes ]), nbfix.itp, silcs.itp, tip3p.itp,
+    forcefield.itp and mol/<name>.pdb/.itp templates of FILE_MOLECULES."""
+    import os
+    os.makedirs(os.path.join(ffdir, 'mol'), exist_ok=True)
+    masses = {t[0]: t[3] for t in FILE_TYPES}
+    with open(os.path.join(ffdir, 'ffnonbonded.itp'), 'w') as f:
+        f.write('[ atomtypes ]\n; name at.num mass charge ptype sigma epsilon\n')
+        for name, rh, eps, mass in FILE_TYPES:
+            s, e = dataset.charmm_to_gromacs(eps, rh)
+            f.write('%-8s %3d %10.4f %6.3f A %.12f %.6f\n' % (name, 6, mass, 0.0, s, e))
+        f.write('\n[ pairtypes ]\n; 1-4 parameters, read as NBFIX by pygcmc\n')
+        for a, b, s, e in pairtypes:
+            f.write('%s %s 1 %g %g\n' % (a, b, s, e))
+    with open(os.path.join(ffdir, 'nbfix.itp'), 'w') as f:
+        f.write('[ nonbond_params ]\n; i j func sigma epsilon\n')
+        for a, b, s, e in nbfix:
+            f.write('%s %s 1 %g %g\n' % (a, b, s, e))
+    with open(os.path.join(ffdir, 'silcs.itp'), 'w') as f:
+        f.write('; no SILCS-specific pairs in this set\n')
+    with open(os.path.join(ffdir, 'forcefield.itp'), 'w') as f:
+        f.write('; (not read by the topology reader: one include level)\n')
+    sol = [a[:3] for a in FILE_MOLECULES['SOL']]
+    with open(os.path.join(ffdir, 'tip3p.itp'), 'w') as f:
+        f.write('\n'.join(['[ moleculetype ]', '; molname nrexcl', 'SOL 2', ''] + _itp_atoms(sol, 'SOL', masses)) + '\n')
+    for res, atoms in FILE_MOLECULES.items():
+        with open(os.path.join(ffdir, 'mol', res.lower() + '.itp'), 'w') as f:
+            f.write('\n'.join(['[ moleculetype ]', f'{res} 3', ''] + _itp_atoms([a[:3] for a in atoms], res, masses)) + '\n')
+        with open(os.path.join(ffdir, 'mol', res.lower() + '.pdb'), 'w') as f:
+            for i, (an, _, _, xyz) in enumerate(atoms):
+                f.write(_pdb_record(i + 1, an, res, 1, [c + 10.0 for c in xyz]) + '\n')
+            f.write('END\n')
+
+
+def write_system_files(outdir, name, box, fixed, molecules, psf=True):
+    """``<name>.pdb``, ``<name>.top`` (and ``<name>.psf``) of a system:
+    ``fixed`` -- list of (atom name, type, charge, residue, resnr, xyz) of one
+    molecule type ``Protein_chain_A``; ``molecules`` -- {species: [centres]}
+    of FILE_MOLECULES placed at those centres, in that species order (after
+    the fixed atoms).  The .top includes ./charmm36.ff/mol/<name>.itp for each
+    species but SOL and ./charmm36.ff/tip3p.itp."""
+    import os
+    os.makedirs(outdir, exist_ok=True)
+    recs = [tuple(r) for r in fixed]
+    resnr = (max(r[4] for r in fixed) + 1) if fixed else 1
+    counts = []
+    for res, centres in molecules.items():
+        tmpl = FILE_MOLECULES[res]
+        for c in centres:
+            for an, ty, q, xyz in tmpl:
+                recs.append((an, ty, q, res, resnr, (c[0] + xyz[0], c[1] + xyz[1], c[2] + xyz[2])))
+            resnr += 1
+        counts.append((res, len(centres)))
+    lines = ['REMARK    synthetic GCMC system',
+             'CRYST1 %8.3f %8.3f %8.3f  90.00  90.00  90.00 P 1           1' % tuple(box)]
+    lines += [_pdb_record(i + 1, an, rn, rnr, xyz) for i, (an, ty, q, rn, rnr, xyz) in enumerate(recs)]
+    with open(os.path.join(outdir, name + '.pdb'), 'w') as f:
+        f.write('\n'.join(lines + ['END']) + '\n')
+    top = ['; synthetic GCMC system', '#include "./charmm36.ff/forcefield.itp"', '',
+           '[ moleculetype ]', '; Name nrexcl', 'Protein_chain_A 3', '',
+           '[ atoms ]', ';   nr  type  resnr residue  atom  cgnr  charge  mass']
+    top += ['%6d %10s %6d %6s %6s %6d %10.3f %10.4f' % (i + 1, ty, rnr, rn, an, i + 1, q, 12.0)
+            for i, (an, ty, q, rn, rnr, _) in enumerate(fixed)]
+    top += ['', '; Include silcs molecules']
+    top += [' #include "./charmm36.ff/mol/%s.itp"' % r.lower() for r in molecules if r != 'SOL']
+    top += ['', '; Include water topology', '#include "./charmm36.ff/tip3p.itp"', '',
+            '#ifdef POSRES_WATER', '[ position_restraints ]', '   1    1       1000       1000       1000', '#endif', '',
+            '[ system ]', name, '', '[ molecules ]', '; Compound        #mols']
+    top += (['Protein_chain_A     1'] if fixed else []) + ['%s\t%d' % rc for rc in counts] + ['']
+    with open(os.path.join(outdir, name + '.top'), 'w') as f:
+        f.write('\n'.join(top))
+    if psf:
+        out = ['PSF EXT', '', '         1 !NTITLE', '* synthetic GCMC system', '', '%10d !NATOM' % len(recs)]
+        out += ['%10d %-8s %-8d %-8s %-8s %-6s %10.6f %13.4f %11d' % (i + 1, 'SYS', rnr, rn, an, ty, q, 12.0, 0)
+                for i, (an, ty, q, rn, rnr, _) in enumerate(recs)]
+        out += ['', '%10d !NBOND: bonds' % 0, '']
+        with open(os.path.join(outdir, name + '.psf'), 'w') as f:
+            f.write('\n'.join(out))
+    return os.path.join(outdir, name)
+
+
+def write_large_files(outdir, n_fixed=30520, n_water=57077, n_meoh=277, box=(138.0, 138.0, 127.0), seed=4000):
+    """A system of the size of pyGCMC's 4wp7 run (30,520 fixed atoms, 57,077
+    waters, fragments; 221k atoms in its log, gcmc.75.out:60-105) in the file
+    formats: alanine residues in a sphere, methanols and waters placed at
+    random (set-up timing only: positions are not equilibrated)."""
+    import os
+    rng = np.random.default_rng(seed)
+    write_forcefield(os.path.join(outdir, 'charmm36.ff'))
+    box = np.asarray(box, np.float64)
+    nres = n_fixed // len(FILE_ALA)
+    ca = random_sphere_atoms(rng, nres, 0.3 * float(box.min()), box, min_dist=3.0)
+    fixed = []
+    for r in range(nres):
+        for j, (an, ty, q) in enumerate(FILE_ALA):
+            fixed.append((an, ty, q, 'ALA', r + 1, tuple(np.round(ca[r] + [0.4 * (j - 5), 0.3 * (j % 3), 0.2 * (j % 2)], 3))))
+    cent = np.round(rng.random((n_water + n_meoh, 3)) * box, 3)
+    return write_system_files(outdir, 'large', box, fixed,
+                              {'MEOH': cent[:n_meoh].tolist(), 'SOL': cent[n_meoh:].tolist()}, psf=False)
