@@ -484,6 +484,9 @@ int compute_layout(gcmc_engine *E)
     l.incg = o; o = up(o + 4 * (l.inc_cap + 1));
     l.incs = o; o = up(o + 4 * (l.inc_cap + 1));
     l.incn = o; o = up(o + 4 * (l.inc_cap + 1));
+    l.map_cap = 4096;
+    l.cand = o; o = up(o + 2 * l.map_cap);
+    l.amap = o; o = up(o + 2 * l.map_cap);
     l.frag = o; o = up(o + 16 * MAXF);
     l.frow = o; o = up(o + 4 * MAXF);
     if (E->staged) {

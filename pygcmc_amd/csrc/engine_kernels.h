@@ -70,6 +70,8 @@ struct Lds {
     double *rsel, *racc, *ea;  // accept: selection / acceptance draw per group, exp term per trial
     int *J;
     int inc_cap;
+    uint16_t *cand, *amap;  // index maps expanded by wg_scan (valid when the total <= map_cap)
+    int map_cap;
 };
 
 __device__ inline Lds carve(char *b, const LdsLayout &l)
@@ -98,6 +100,9 @@ __device__ inline Lds carve(char *b, const LdsLayout &l)
     s.ea = (double *)(b + l.ea);
     s.J = (int *)(b + l.J);
     s.inc_cap = l.inc_cap;
+    s.cand = (uint16_t *)(b + l.cand);
+    s.amap = (uint16_t *)(b + l.amap);
+    s.map_cap = l.map_cap;
     return s;
 }
 
@@ -109,7 +114,10 @@ __device__ inline void wave_sync()
 
 // exclusive scan of a[0..n) in LDS, a[n] = total; returns total
 template <int NT>
-__device__ inline int wg_scan(int *a, int n, int *misc)
+// When map is given and the total is at most cap, also writes map[j] = i for
+// every j in [a'[i], a'[i+1]) (a' the scanned array): the entry each item
+// belongs to, so a thread finds it with one read instead of a binary search.
+__device__ inline int wg_scan(int *a, int n, int *misc, uint16_t *map = nullptr, int cap = 0)
 {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -137,12 +145,15 @@ __device__ inline int wg_scan(int *a, int n, int *misc)
     }
     __syncthreads();
     int base = misc[M_SCAN + wave] + x - sum;
+    const int total = misc[M_SCAN + NW];
+    const bool expand = map != nullptr && total <= cap;
     for (int i = b; i < e; ++i) {
         int v = a[i];
         a[i] = base;
+        if (expand)
+            for (int j = 0; j < v; ++j) map[base + j] = (uint16_t)i;
         base += v;
     }
-    const int total = misc[M_SCAN + NW];
     if (tid == 0) a[n] = total;
     __syncthreads();
     return total;
@@ -196,7 +207,8 @@ __device__ inline float two_diff(float a, float b, float &e)
 template <int NT, bool STAGED, bool EW>
 __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
 {
-    const int A = wg_scan<NT>(L.incn, n, L.misc);
+    const int A = wg_scan<NT>(L.incn, n, L.misc, L.amap, L.map_cap);
+    const bool mapped = A <= L.map_cap;
     const auto atom = gbl(C.atom);
     const auto atype = gbl(C.atype);
     const auto GPT = gbl(C.pair);
@@ -281,7 +293,12 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
         const int t1 = t0 + NT;
         const bool two = t1 < A;
         int g0, g1;
-        bsearch_le2(L.incn, n, t0, two ? t1 : t0, g0, g1);
+        if (mapped) {
+            g0 = L.amap[t0];
+            g1 = two ? L.amap[t1] : g0;
+        } else {
+            bsearch_le2(L.incn, n, t0, two ? t1 : t0, g0, g1);
+        }
         const int a0 = L.incs[g0] + (t0 - L.incn[g0]);
         const int a1 = two ? L.incs[g1] + (t1 - L.incn[g1]) : a0;
         const float4 at0 = atom[a0];
@@ -381,7 +398,8 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
         L.soff[e] = fc + mc;
     }
     __syncthreads();
-    const int M = wg_scan<NT>(L.soff, ne, L.misc);
+    const int M = wg_scan<NT>(L.soff, ne, L.misc, L.cand, L.map_cap);
+    const bool mapped = M <= L.map_cap;
     const unsigned long long tk1 = wall_clock64();
     float acc = 0.0f, cmp = 0.0f;
     double acc64 = 0.0;  // FP64 pair terms (Coulomb)
@@ -395,7 +413,7 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
         int gid = -1;
         int2 rr = make_int2(0, 0);
         if (m < M) {
-            const int e = bsearch_le(L.soff, ne, m);
+            const int e = mapped ? (int)L.cand[m] : bsearch_le(L.soff, ne, m);
             const int local = m - L.soff[e];
             const int fc = L.sfix[e];
             float4 p;

@@ -139,6 +139,7 @@ struct LdsLayout {
     int rsel, racc, ea;            // accept: double[maxk] each -- draws per group, exp terms per trial
     int J;                         // prep: int[maxk]
     int inc_cap;                   // included groups buffered before a pair pass
+    int cand, amap, map_cap;       // uint16[map_cap] each: candidate -> stencil entry, pair-pass atom -> group
     int total;
 };
 
