@@ -15,6 +15,9 @@ format readers and writers (run in a tree where the reference is at
                               first 30 atoms, the atom count line set to 30
 * ref_psftest_head.pdb     -- the first 30 atom records of the matching .pdb
                               with its CRYST1 record
+* ref_4wp7_gcmc.75.inp     -- test/4wp7/gcmc.75.inp, a legacy parameter file
+* ref_4wp7_params.out      -- the lines of test/4wp7/gcmc.75.out where the
+                              legacy command echoed the parsed parameters
 """
 import os
 
@@ -53,6 +56,16 @@ def main():
     at = [ln for ln in pdb if ln.startswith('ATOM')][:30]
     with open(os.path.join(OUT, 'ref_psftest_head.pdb'), 'w') as f:
         f.write('\n'.join(cr + at + ['END']) + '\n')
+    w = os.path.join(REF, '4wp7')
+    with open(os.path.join(w, 'gcmc.75.inp')) as f:
+        inp = f.read()
+    with open(os.path.join(OUT, 'ref_4wp7_gcmc.75.inp'), 'w') as f:
+        f.write(inp)
+    log = lines(os.path.join(w, 'gcmc.75.out'))
+    first = next(i for i, ln in enumerate(log) if ln.startswith('par ['))
+    last = next(i for i, ln in enumerate(log) if ln.startswith('Using pdb file'))
+    with open(os.path.join(OUT, 'ref_4wp7_params.out'), 'w') as f:
+        f.write('\n'.join(log[first:last]) + '\n')
 
 
 if __name__ == '__main__':

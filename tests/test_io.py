@@ -230,3 +230,45 @@ def test_run_with_oracle_and_outputs(tmp_path):
     assert f'\nMEOH\t{n[0]}\n' in sect and f'\nSOL\t{n[1]}\n' in sect
     re_atoms, _ = formats.parse_top(top, MINI)
     assert len(re_atoms) == len(back)
+
+
+# -- legacy parameter-file command ---------------------------------------------------
+def test_legacy_parameter_file_matches_reference_log():
+    """test/4wp7/gcmc.75.inp parsed as pyGCMC's gcmc command did: the same
+    key/value lines its log printed (gcmc.75.out)."""
+    from pygcmc_amd import legacy
+    cfg = legacy.load_parameters(os.path.join(IO, 'ref_4wp7_gcmc.75.inp'))
+    printed = [f'{k} {v}' for k, v in cfg.items()]
+    expected = open(os.path.join(IO, 'ref_4wp7_params.out')).read().rstrip('\n').split('\n')
+    assert printed == expected
+    assert cfg['par'] == ['charmm36.ff/ffnonbonded.itp', 'charmm36.ff/silcs.itp', 'charmm36.ff/nbfix.itp']
+    assert [legacy.true_value(v) for v in ('yes', 'True', 'ON', '1', 'no', '0', 'off')] == [True] * 4 + [False] * 3
+
+
+def test_legacy_command(tmp_path):
+    import shutil
+    from oracle import oracle as O
+    from pygcmc_amd import legacy
+    for f in ('mini.pdb', 'mini.top'):
+        shutil.copy(os.path.join(MINI, f), tmp_path / f)
+    inp = tmp_path / 'run.inp'
+    inp.write_text('\n'.join([
+        f'pdb:{tmp_path / "mini.pdb"}', f'top:{tmp_path / "mini.top"}',
+        f'op_pdb:{tmp_path / "out.pdb"}', f'op_top:{tmp_path / "out.top"}   # written by the run',
+        'grid_dx:   1.000', 'use_cavity_bias:yes', 'random_seed:166 // comment', 'map_generation:yes',
+        'fragname:     meoh      sol ', 'fragconc:     0.25    55.00 ', 'fragmuex:    -2.04    -5.04 ',
+        'fragconf:       10        2 ', 'mctime:      0.2      0.8 ', 'mcsteps:200', '']))
+    rc = legacy.main(['-p', str(inp), '--ff-dir', FF, '--logfile', str(tmp_path / 'log')],
+                     runner=lambda *a: O.run(dict(zip(KEYS, a))))
+    assert rc == 0
+    log = (tmp_path / 'log').read_text()
+    assert 'Fragment meoh:\tmuex: -2.04\tconf: 10\tconc: 0.25\tmctime: 0.2' in log
+    out = formats.read_pdb(str(tmp_path / 'out.pdb'))
+    atoms, _ = formats.parse_top((tmp_path / 'out.top').read_text(), MINI)
+    assert len(atoms) == len(out)
+    # same run through GCMCSystem gives the same output
+    s = GCMCSystem(fragment_names=['MEOH', 'SOL'], conc=[0.25, 55.0], muex=[-2.04, -5.04], conf=[10, 2],
+                   mctime=[0.2, 0.8], mcsteps=200, seed=166, ff_dir=FF)
+    s.load(str(tmp_path / 'mini.pdb'), top=str(tmp_path / 'mini.top')).setup()
+    s.run(runner=lambda *a: O.run(dict(zip(KEYS, a))))
+    assert s.output_pdb() == (tmp_path / 'out.pdb').read_text()
