@@ -173,6 +173,17 @@ int gcmc_engine_eval_poses(gcmc_engine *e, int replica, int species, int npose,
                            const float *centres, const float *poses, const int *exclude,
                            double *energy, int *ninc, uint32_t *inchash);
 
+/* ---- occupancy maps (no counterpart in the reference: its map_generation
+ * key is parsed and ignored, gcmc/scripts/mainOld.py) ----------------------
+ * Adds to counts[n0][n1][n2] (a DEVICE uint32 array) one count per atom of
+ * every molecule of `species` in the replica's current state, in the cell
+ * int((x - origin)/spacing) mod n per axis (truncation toward zero,
+ * non-negative remainder, FP64: the cavity grid's rule).  Queued on the
+ * engine's stream after the runs queued before it; read counts after
+ * gcmc_engine_synchronize. */
+int gcmc_engine_occupancy(gcmc_engine *e, int replica, int species, const double *origin, double spacing,
+                          const int *n, unsigned int *counts);
+
 /* ---- adjoint of the pose energy (no counterpart in the reference) ----------
  * Energy of npose poses of one species (all groups of the replica's current
  * state whose centre passes the group test around centres[i]; nothing

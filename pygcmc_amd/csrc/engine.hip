@@ -1129,6 +1129,52 @@ int gcmc_engine_eval_poses(gcmc_engine *E, int r, int species, int npose, const 
     return 0;
 }
 
+namespace {
+// cell of a coordinate along one axis: truncation toward zero, non-negative
+// remainder (the cavity grid's rule, gcmc/python/dataset.py:65-68), FP64
+__device__ inline long long occ_cell(double x, double o, double dx, int n)
+{
+    const long long q = (long long)trunc((x - o) / dx);
+    const long long m = q % n;
+    return m < 0 ? m + n : m;
+}
+
+// one thread per (slot, atom) of species spi; slots at or beyond the
+// species' current count exit
+__global__ __launch_bounds__(256) void k_occupancy(const Rep *reps, const State *st, int r, int spi, double o0,
+                                                   double o1, double o2, double dx, int n0, int n1, int n2,
+                                                   unsigned int *counts)
+{
+    const Rep &C = reps[r];
+    const Species &F = C.sp[spi];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)st[r].totalNum[spi] * F.nf) return;
+    const float4 p = C.atom[F.atomBase + i];
+    const long long lin = (occ_cell((double)p.x, o0, dx, n0) * n1 + occ_cell((double)p.y, o1, dx, n1)) * n2 +
+                          occ_cell((double)p.z, o2, dx, n2);
+    atomicAdd(&counts[lin], 1u);
+}
+}  // namespace
+
+int gcmc_engine_occupancy(gcmc_engine *E, int r, int species, const double *origin, double spacing,
+                          const int *n, unsigned int *counts)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (r < 0 || r >= (int)E->reps.size()) return fail(GCMC_E_ARG, "bad replica");
+    HostRep &H = *E->reps[r];
+    if (species < 0 || species >= H.info.fragTypeNum) return fail(GCMC_E_ARG, "bad species");
+    if (!origin || !n || !counts) return fail(GCMC_E_ARG, "null array");
+    if (!(spacing > 0.0)) return fail(GCMC_E_ARG, "spacing must be positive");
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1 || (long long)n[0] * n[1] * n[2] > (1ll << 31))
+        return fail(GCMC_E_ARG, "grid dimensions must be 1..2^31 cells in total");
+    const long long slots = (long long)H.frag[species].maxNum * H.frag[species].num_atoms;
+    if (slots == 0) return 0;
+    hipLaunchKernelGGL(k_occupancy, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, E->stream, E->d_rep,
+                       E->d_state, r, species, origin[0], origin[1], origin[2], spacing, n[0], n[1], n[2], counts);
+    HIPX(hipGetLastError());
+    return 0;
+}
+
 int gcmc_engine_set_ewald(gcmc_engine *E, int r, float beta, float grid_spacing)
 {
     if (!E) return fail(GCMC_E_ARG, "null engine");

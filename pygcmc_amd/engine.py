@@ -197,6 +197,10 @@ class Engine:
         _check(lib().gcmc_engine_run(self._h, int(nsteps), MODES[mode]))
         return self.last_run_ms()
 
+    def synchronize(self):
+        """Wait for everything queued on the engine's stream."""
+        _check(lib().gcmc_engine_synchronize(self._h))
+
     def last_run_ms(self):
         ms = ctypes.c_double()
         _check(lib().gcmc_engine_last_run_ms(self._h, ctypes.byref(ms)))

@@ -3,11 +3,14 @@
 file.
 
 Keys read (the rest -- ``par``, ``fragitp``, ``monomerdir``, ``box_size``,
-cut-offs, ``map_generation``, ... -- are parsed and ignored, as pyGCMC does):
+cut-offs, ... -- are parsed and ignored, as pyGCMC does):
 ``pdb``, ``top``, ``op_pdb``, ``op_top``, ``fragname`` (upper-cased),
 ``fragmuex``, ``fragconf``, ``fragconc``, ``mctime`` (whitespace-separated
 per species), ``mcsteps``, ``grid_dx``, ``use_cavity_bias`` (absent = no),
-``random_seed``, ``PME``.  Writes ``op_pdb`` and ``op_top``.  Added:
+``random_seed``, ``PME``, and ``map_generation`` / ``map_filename_prefix``,
+which pyGCMC ignores: here they run the chain in 100 chunks with an
+occupancy snapshot after each and write ``<prefix>.<species>.gfe.dx``
+(``pygcmc_amd.maps``).  Writes ``op_pdb`` and ``op_top``.  Added:
 ``--ff-dir`` (force-field directory, default ``charmm36.ff``).
 """
 
@@ -85,7 +88,12 @@ def main(argv=None, runner=None):
         s.load(cfg['pdb'][0], top=cfg['top'][0])
         s.setup()
         print(f'Python time used: {time.time() - t0} s')
-        s.run(runner)
+        if true_value(cfg.get('map_generation', ['no'])[0]) and runner is None:
+            prefix = cfg.get('map_filename_prefix', [cfg['op_pdb'][0].rsplit('.', 1)[0]])[0]
+            s.run_maps(every=max(1, s.mcsteps // 100), spacing=s.grid_dx if s.cavity_bias else 1.0, prefix=prefix)
+            print(f'maps: {prefix}.<species>.gfe.dx')
+        else:
+            s.run(runner)
         print(f'GPU simulation time: {s.timings["run_s"]:.2f} s')
         s.write(cfg['op_pdb'][0], cfg['op_top'][0])
         print(f'Time used: {time.time() - t0} s')

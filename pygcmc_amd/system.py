@@ -247,6 +247,42 @@ class GCMCSystem:
         self.timings['run_s'] = time.perf_counter() - t0
         return self
 
+    def run_maps(self, every=100, spacing=1.0, prefix=None, cap=12.552, tuning=None):
+        """Run on the engine in chunks of ``every`` moves (batched form; the
+        same chain as ``run``), adding one occupancy snapshot per species
+        after each chunk (``pygcmc_amd.maps``).  With ``prefix``, writes
+        ``<prefix>.<species>.gfe.dx``: grid free energies in kJ/mol, empty
+        cells at ``cap`` (default 3 kcal/mol).  Returns the maps."""
+        from .engine import Engine
+        from .maps import OccupancyMap
+        if self.arrays is None:
+            self.setup()
+        t0 = time.perf_counter()
+        sysd = dict(zip(('info', 'frag', 'res', 'atoms', 'grid', 'ff', 'moves'), self.arrays))
+        with Engine(**(tuning or {})) as eng:
+            eng.add_system(sysd)
+            eng.upload()
+            maps = [OccupancyMap(eng, 0, sp, spacing) for sp in range(len(self.names))]
+            done = 0
+            while done < self.mcsteps:
+                n = min(int(every), self.mcsteps - done)
+                eng.run(n, 'batched')
+                done += n
+                for m in maps:
+                    m.accumulate()
+            for m in maps:
+                m.freeze()
+            eng.download(0)
+        self.timings['run_s'] = time.perf_counter() - t0
+        if prefix:
+            d = os.path.dirname(prefix)
+            if d:
+                os.makedirs(d, exist_ok=True)
+            for name, m in zip(self.names, maps):
+                m.write_dx(f'{prefix}.{name.lower()}.gfe.dx', m.free_energy(cap=cap))
+        self.maps = maps
+        return maps
+
     def counts(self):
         frag = self.arrays[1]
         return [(self.names[s], int(frag[s]['totalNum'])) for s in range(len(self.names))]
