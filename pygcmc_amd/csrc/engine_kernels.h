@@ -112,6 +112,105 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// ---- wave-wide scans on DPP (every lane of the wave active) -----------------
+// Inclusive scan: Hillis-Steele within each row of 16 lanes (row_shr 1, 2, 4,
+// 8), then row 0's last lane into row 1 and row 2's into row 3 (row_bcast:15),
+// then lane 31 into rows 2 and 3 (row_bcast:31).  Lanes without a source keep
+// the identity.  Register moves between lanes instead of ds_bpermute round
+// trips through the LDS crossbar.
+enum : int { DPP_SHR1 = 0x111, DPP_SHR2 = 0x112, DPP_SHR4 = 0x114, DPP_SHR8 = 0x118, DPP_BC15 = 0x142, DPP_BC31 = 0x143 };
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_i(int identity, int v)
+{
+    return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWS, 0xf, false);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double identity, double v)
+{
+    const long long b = __double_as_longlong(v), o = __double_as_longlong(identity);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <class T, class Op>
+__device__ __forceinline__ T wave_scan(T x, T id, Op op)
+{
+    if constexpr (sizeof(T) == 8) {
+        x = op(x, dpp_d<DPP_SHR1, 0xf>(id, x));
+        x = op(x, dpp_d<DPP_SHR2, 0xf>(id, x));
+        x = op(x, dpp_d<DPP_SHR4, 0xf>(id, x));
+        x = op(x, dpp_d<DPP_SHR8, 0xf>(id, x));
+        x = op(x, dpp_d<DPP_BC15, 0xa>(id, x));
+        x = op(x, dpp_d<DPP_BC31, 0xc>(id, x));
+    } else {
+        x = op(x, (T)dpp_i<DPP_SHR1, 0xf>((int)id, (int)x));
+        x = op(x, (T)dpp_i<DPP_SHR2, 0xf>((int)id, (int)x));
+        x = op(x, (T)dpp_i<DPP_SHR4, 0xf>((int)id, (int)x));
+        x = op(x, (T)dpp_i<DPP_SHR8, 0xf>((int)id, (int)x));
+        x = op(x, (T)dpp_i<DPP_BC15, 0xa>((int)id, (int)x));
+        x = op(x, (T)dpp_i<DPP_BC31, 0xc>((int)id, (int)x));
+    }
+    return x;
+}
+
+__device__ __forceinline__ int wave_scan_add(int x)
+{
+    return wave_scan(x, 0, [](int a, int b) { return a + b; });
+}
+__device__ __forceinline__ uint32_t wave_scan_add(uint32_t x)
+{
+    return wave_scan(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+__device__ __forceinline__ double wave_scan_add(double x)
+{
+    return wave_scan(x, 0.0, [](double a, double b) { return a + b; });
+}
+// minimum over the wave, in every lane
+__device__ __forceinline__ double wave_min(double x)
+{
+    const double m = wave_scan(x, __builtin_huge_val(), [](double a, double b) { return b < a ? b : a; });
+    const long long bm = __double_as_longlong(m);
+    const int lo = __builtin_amdgcn_readlane((int)bm, 63), hi = __builtin_amdgcn_readlane((int)(bm >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ unsigned wave_min(unsigned x)
+{
+    const unsigned m = wave_scan(x, 0xFFFFFFFFu, [](unsigned a, unsigned b) { return b < a ? b : a; });
+    return (unsigned)__builtin_amdgcn_readlane((int)m, 63);
+}
+
+// 64-bit wave values by lane index (idx uniform across the wave)
+__device__ inline double lane_d(double v, int idx)
+{
+    const int i = __builtin_amdgcn_readfirstlane(idx);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), i);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), i);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline unsigned long long lane_u64(unsigned long long v, int idx)
+{
+    const int i = __builtin_amdgcn_readfirstlane(idx);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, i);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), i);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ inline int lane_i(int v, int idx)
+{
+    return __builtin_amdgcn_readlane(v, __builtin_amdgcn_readfirstlane(idx));
+}
+// lane r receives v of the lane whose dst is r (dst a permutation of 0..63)
+__device__ inline double permute_d(int dst, double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_permute(dst * 4, (int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_ds_permute(dst * 4, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // exclusive scan of a[0..n) in LDS, a[n] = total; returns total
 template <int NT>
 // When map is given and the total is at most cap, also writes map[j] = i for
@@ -126,12 +225,7 @@ __device__ inline int wg_scan(int *a, int n, int *misc, uint16_t *map = nullptr,
     const int e = min(n, b + per);
     int sum = 0;
     for (int i = b; i < e; ++i) sum += a[i];
-    int x = sum;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
+    const int x = wave_scan_add(sum);
     if (lane == 63) misc[M_SCAN + wave] = x;
     __syncthreads();
     if (tid == 0) {
@@ -465,14 +559,10 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
             __syncthreads();
         }
     }
-    double v = (double)acc + (double)cmp + acc64;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        v += __shfl_down(v, off, 64);
-        nin += __shfl_down(nin, off, 64);
-        hs += __shfl_down(hs, off, 64);
-    }
-    if (lane == 0) {
+    const double v = wave_scan_add((double)acc + (double)cmp + acc64);
+    nin = wave_scan_add(nin);
+    hs = wave_scan_add(hs);
+    if (lane == 63) {
         L.dred[wave] = v;
         L.misc[M_NIN + wave] = nin;
         L.misc[M_HASH + wave] = (int)hs;
@@ -837,13 +927,8 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
         if (lane == (it >> 5)) m |= 1u << (it & 31);
         u &= ~m;
         const int cnt = __popc(m);
-        int x = cnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
-        const int mtot = __shfl(x, 63, 64);
+        const int x = wave_scan_add(cnt);
+        const int mtot = __builtin_amdgcn_readlane(x, 63);
         int o = x - cnt;
         uint32_t mm = m;
         while (mm) {
@@ -862,11 +947,7 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
                 const double v = L.val[L.members[q]];
                 emin = v < emin ? v : emin;
             }
-    #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double o2 = __shfl_xor(emin, off, 64);
-                emin = o2 < emin ? o2 : emin;
-            }
+            emin = wave_min(emin);
             for (int q = lane; q < mtot; q += 64) L.w[q] = exp(-BETA * (L.val[L.members[q]] - emin));
             wave_sync();
             // chunk sums in member order, scanned across lanes
@@ -874,13 +955,8 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
             const int q0 = min(lane * per, mtot), q1 = min(q0 + per, mtot);
             double part = 0.0;
             for (int q = q0; q < q1; ++q) part += L.w[q];
-            double incl = part;
-    #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const double y = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += y;
-            }
-            sumP = __shfl(incl, 63, 64);
+            const double incl = wave_scan_add(part);
+            sumP = lane_d(incl, 63);
             if (sumP != 0.0) {
                 // first member q with sum_{j<=q} w_j > ran_sel * sumP (the last one
                 // when rounding leaves the target at the total)
@@ -952,35 +1028,6 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
     }
 }
 
-// 64-bit wave values by lane index (idx uniform across the wave)
-__device__ inline double lane_d(double v, int idx)
-{
-    const int i = __builtin_amdgcn_readfirstlane(idx);
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), i);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), i);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-__device__ inline unsigned long long lane_u64(unsigned long long v, int idx)
-{
-    const int i = __builtin_amdgcn_readfirstlane(idx);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, i);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), i);
-    return ((unsigned long long)hi << 32) | lo;
-}
-__device__ inline int lane_i(int v, int idx)
-{
-    return __builtin_amdgcn_readlane(v, __builtin_amdgcn_readfirstlane(idx));
-}
-// lane r receives v of the lane whose dst is r (dst a permutation of 0..63)
-__device__ inline double permute_d(int dst, double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_ds_permute(dst * 4, (int)(b & 0xffffffffll));
-    const int hi = __builtin_amdgcn_ds_permute(dst * 4, (int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // greedy_select for k <= 64 with the per-trial values in registers: lane t
 // holds trial t's value, exp term, adjacency row (bit j: trial j within the
 // cutoff) and slot, and group t's two variates.  Group members are compacted
@@ -1011,23 +1058,13 @@ __device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int
         double P = 1.0;
         if (mtot > 1) {
             const bool in = ((M >> lane) & 1ull) != 0ull;
-            double emin = in ? val : __builtin_huge_val();
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double o2 = __shfl_xor(emin, off, 64);
-                emin = o2 < emin ? o2 : emin;
-            }
+            const double emin = wave_min(in ? val : __builtin_huge_val());
             const double wl = in ? exp(-BETA * (val - emin)) : 0.0;
             const int dst = in ? __popcll(M & below) : mtot + __popcll(~M & below);
             const double w = permute_d(dst, wl);                      // lane q: member q's weight
             const int tix = __builtin_amdgcn_ds_permute(dst * 4, lane);  // lane q: member q's trial
             const double part = lane < mtot ? w : 0.0;
-            double incl = part;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const double y = __shfl_up(incl, off, 64);
-                if (lane >= off) incl += y;
-            }
+            const double incl = wave_scan_add(part);
             const double sumP = lane_d(incl, 63);
             int pq = 0;
             if (sumP != 0.0) {
@@ -1518,12 +1555,8 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
                     const unsigned size = unc > 255ull ? 255u : (unsigned)unc;
                     const unsigned prio = (unsigned)(w >> 40) >> 8;
                     while (__ballot(avail) != 0ull) {
-                        unsigned key = avail ? ((size << 22) | (prio << 6) | (unsigned)lane) : 0xFFFFFFFFu;
-#pragma unroll
-                        for (int off = 32; off > 0; off >>= 1) {
-                            const unsigned o2 = __shfl_xor(key, off, 64);
-                            key = o2 < key ? o2 : key;
-                        }
+                        const unsigned key =
+                            wave_min(avail ? ((size << 22) | (prio << 6) | (unsigned)lane) : 0xFFFFFFFFu);
                         const int l = (int)(key & 63u);
                         const int rr = __shfl(r, l, 64);
                         unsigned long long old = 0ull;
