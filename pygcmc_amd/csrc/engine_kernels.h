@@ -53,6 +53,8 @@ enum Misc : int {
     M_FOUND = 88,    // batched: replica claimed (-1 none, -2 exit)
     M_ITEM = 89,     //          item claimed
     M_LAST = 90,     //          this workgroup finished the move's last item
+    M_NX = 91,       // [91, 97)  next move, loaded during the accept phase:
+                     //           end step, move, species count, conf, maxNum, startRes
 };
 
 struct Lds {
@@ -900,11 +902,53 @@ __device__ inline uint32_t adj_word(const Rep &C, const Lds &L, int nw, int t, i
     return L.adjl ? L.adjl[t * nw + w] : C.adj[t * ADJW + w];
 }
 
-__device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int row, bool trace)
+// A move's descriptor and the replica values its accept phase reads, loaded
+// once at its start: every later phase would otherwise reload them after a
+// barrier, one dependent L2 round trip each on the replica's serial path.
+struct MoveCtx {
+    int kind, spi, k, step, nf, N, maxN, startRes, atomBase;
+    double B, energy;
+};
+
+__device__ inline MoveCtx move_ctx(const Rep &C, const State &S)
+{
+    MoveCtx m;
+    m.kind = S.kind;
+    m.spi = S.sp;
+    m.k = S.k;
+    m.step = S.step;
+    m.N = S.totalNum[m.spi];
+    m.energy = S.energy;
+    const auto &F = gbl(C.sp)[m.spi];
+    m.nf = F.nf;
+    m.maxN = F.maxNum;
+    m.startRes = F.startRes;
+    m.atomBase = F.atomBase;
+    m.B = F.B;
+    return m;
+}
+
+// the next move's descriptor into L.misc[M_NX..] (one thread; the accept
+// phase's selection wave is busy meanwhile).  The species count is the one
+// before the current move: prep takes the current move's result for its own
+// species.
+__device__ inline void load_next_move(const Rep &C, const State &S, const Lds &L, const MoveCtx &m)
+{
+    const int end = S.end, step = m.step + 1;
+    const int mv = step < end ? C.moves[step] : 0;
+    const auto &F = gbl(C.sp)[mv >> 2];
+    L.misc[M_NX + 0] = end;
+    L.misc[M_NX + 1] = mv;
+    L.misc[M_NX + 2] = S.totalNum[mv >> 2];
+    L.misc[M_NX + 3] = F.conf;
+    L.misc[M_NX + 4] = F.maxNum;
+    L.misc[M_NX + 5] = F.startRes;
+}
+
+__device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc)
 {
     const int lane = threadIdx.x & 63;
-    const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
-    const Species &F = C.sp[spi];
+    const int kind = mc.kind, k = mc.k;
     const bool ins = kind == K_INS;
     const int nw = (k + 31) >> 5;
     uint32_t u = 0u;
@@ -912,11 +956,11 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
         const int hi = k - lane * 32;
         u = hi >= 32 ? 0xffffffffu : ((1u << hi) - 1u);
     }
-    int N = S.totalNum[spi];
-    const int maxN = F.maxNum;
+    int N = mc.N;
+    const int maxN = mc.maxN;
     const double cav = (double)C.cav;
     int g = 0, na = 0;
-    double et = S.energy;
+    double et = mc.energy;
     while (true) {
         const unsigned long long bal = __ballot(u != 0u);
         if (bal == 0ull) break;
@@ -1024,7 +1068,7 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
         L.misc[M_NG] = g;
         L.misc[M_N] = N;
         S.energy = et;
-        S.acc[spi * 4 + kind] += na;
+        S.acc[mc.spi * 4 + kind] += na;
     }
 }
 
@@ -1033,20 +1077,20 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
 // cutoff) and slot, and group t's two variates.  Group members are compacted
 // into lanes 0..m-1 in trial order (ds_permute), so the weights, their sums
 // and the decisions are those of greedy_select, bit for bit.  Wave 0 only.
-__device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int row, bool trace, double val,
-                                       double ea, double rsel, double racc, unsigned long long adj, int tslot)
+__device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc,
+                                       double val, double ea, double rsel, double racc, unsigned long long adj,
+                                       int tslot)
 {
     const int lane = threadIdx.x & 63;
-    const int kind = S.kind, spi = S.sp, k = S.k;
-    const Species &F = C.sp[spi];
+    const int kind = mc.kind, k = mc.k;
     const bool ins = kind == K_INS;
     const unsigned long long below = (1ull << lane) - 1ull;
     unsigned long long U = k >= 64 ? ~0ull : ((1ull << k) - 1ull);
-    int N = S.totalNum[spi];
-    const int maxN = F.maxNum;
+    int N = mc.N;
+    const int maxN = mc.maxN;
     const double cav = (double)C.cav;
     int g = 0, na = 0;
-    double et = S.energy;
+    double et = mc.energy;
     while (U != 0ull) {
         const int it = __ffsll((long long)U) - 1;
         const unsigned long long M = (lane_u64(adj, it) & U) | (1ull << it);
@@ -1115,7 +1159,7 @@ __device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int
         L.misc[M_NG] = g;
         L.misc[M_N] = N;
         S.energy = et;
-        S.acc[spi * 4 + kind] += na;
+        S.acc[mc.spi * 4 + kind] += na;
     }
 }
 
@@ -1123,18 +1167,18 @@ __device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int
 // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses, then the group
 // centres and cell-list entries in acceptance order (thread 0).
 template <int NT, bool EW>
-__device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int row, bool trace)
+__device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc)
 {
     const int tid = threadIdx.x;
-    const int kind = S.kind, spi = S.sp;
+    const int kind = mc.kind, spi = mc.spi;
     const Species &F = C.sp[spi];
-    const int nf = F.nf;
+    const int nf = mc.nf;
     const int na = L.misc[M_ACC];
     // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses
     for (int idx = tid; idx < na * nf; idx += NT) {
         const int q = idx / nf, a = idx - q * nf;
         const int2 ts = L.accl[q];
-        const int base = F.atomBase + ts.y * nf;
+        const int base = mc.atomBase + ts.y * nf;
         const float4 p = C.pose[ts.x * MAXF + a];
         float4 dst;
         if (kind == K_INS) {
@@ -1155,7 +1199,7 @@ __device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int
     if (tid == 0) {
         for (int q = 0; q < na; ++q) {
             const int2 ts = L.accl[q];
-            const int gid = F.startRes + ts.y;
+            const int gid = mc.startRes + ts.y;
             const float4 c = C.tc[ts.x];
             if (kind == K_INS) {
                 C.res_pos[gid] = make_float4(c.x, c.y, c.z, 0.f);
@@ -1174,12 +1218,12 @@ __device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int
 
 // ---- accept phase: decisions and commit ------------------------------------------
 template <int NT, bool EW>
-__device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
+__device__ inline void accept_move(const Rep &C, State &S, const Lds &L, const MoveCtx &mc, bool prefetch = false)
 {
     const int tid = threadIdx.x;
-    const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
+    const int kind = mc.kind, spi = mc.spi, k = mc.k, step = mc.step;
     const Species &F = C.sp[spi];
-    const int nf = F.nf;
+    const int nf = mc.nf;
     int row;
     const bool trace = tracing(C, step, row);
     // stage the trial values (E for insert/delete, E_new - E_old otherwise);
@@ -1191,16 +1235,17 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
         S.att[spi * 4 + kind] += 1;
         L.misc[M_ACC] = 0;
         L.misc[M_NG] = 0;
-        L.misc[M_N] = S.totalNum[spi];
+        L.misc[M_N] = mc.N;
     }
     __syncthreads();
+    if (prefetch && tid == 64) load_next_move(C, S, L, mc);
     if (kind == K_DEL && k == 1) {
-        const int N = S.totalNum[spi];
+        const int N = mc.N;
         if (tid == 0) {
             const double E = L.val[0];
             const u32x4 x = draw(C.seed, (uint32_t)step, 0u, P_DECIDE);
             const double ran = u01d(x.z, x.w);
-            const double pp = (double)N * exp(-F.B - BETA * (-E));
+            const double pp = (double)N * exp(-mc.B - BETA * (-E));
             const double p = pp < 1.0 ? pp : 1.0;
             const int acc = ran < p;
             L.misc[M_ACC] = acc;
@@ -1210,7 +1255,7 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
                 C.tr.gacc[row * C.tr.max_k] = acc;
             }
             if (acc) {
-                S.energy -= E;
+                S.energy = mc.energy - E;
                 S.acc[spi * 4 + kind] += 1;
             }
         }
@@ -1269,12 +1314,12 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
                 const u32x4 xr = draw(C.seed, (uint32_t)step, (uint32_t)t, P_DECIDE);
                 rsel = u01d(xr.x, xr.y);
                 racc = u01d(xr.z, xr.w);
-                ea = exp(((kind == K_INS) ? F.B : 0.0) - BETA * val);
+                ea = exp(((kind == K_INS) ? mc.B : 0.0) - BETA * val);
             }
-            greedy_select64(C, S, L, row, trace, val, ea, rsel, racc, adj, sl);
+            greedy_select64(C, S, L, row, trace, mc, val, ea, rsel, racc, adj, sl);
         }
         __syncthreads();
-        commit_accepted<NT, EW>(C, S, L, row, trace);
+        commit_accepted<NT, EW>(C, S, L, row, trace, mc);
     } else if (k > 0 && kind != K_DEL) {
         if (L.adjl) {
             const int nw = (k + 31) >> 5;
@@ -1287,7 +1332,7 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
         // groups: the variates of group g (at most k groups) and each trial's
         // exp(B - beta E) (insert) or exp(-beta dE) (move)
         {
-            const double Bx = (kind == K_INS) ? F.B : 0.0;
+            const double Bx = (kind == K_INS) ? mc.B : 0.0;
             for (int t = tid; t < k; t += NT) {
                 const u32x4 xr = draw(C.seed, (uint32_t)step, (uint32_t)t, P_DECIDE);
                 L.rsel[t] = u01d(xr.x, xr.y);
@@ -1296,9 +1341,9 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
             }
         }
         __syncthreads();
-        if ((tid >> 6) == 0) greedy_select(C, S, L, row, trace);
+        if ((tid >> 6) == 0) greedy_select(C, S, L, row, trace, mc);
         __syncthreads();
-        commit_accepted<NT, EW>(C, S, L, row, trace);
+        commit_accepted<NT, EW>(C, S, L, row, trace, mc);
     }
     if (tid == 0) {
         if (trace) {
@@ -1313,20 +1358,43 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
     __syncthreads();
 }
 
+template <int NT, bool EW>
+__device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
+{
+    const MoveCtx mc = move_ctx(C, S);
+    accept_move<NT, EW>(C, S, L, mc);
+}
+
 // ---- prep phase: picks and trial centres of S.step ------------------------------
+// pre: the move's descriptor loaded with the previous move (for the same
+// species, the count that move's accept phase left in L.misc[M_N]), or null
+// use_pre: the move after prev (species prev_spi) was loaded into L.misc[M_NX..]
 template <int NT>
-__device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
+__device__ inline int prep_step(const Rep &C, State &S, const Lds &L, bool use_pre, int prev_step, int prev_spi)
 {
     const int tid = threadIdx.x;
-    const int step = S.step;
-    const int mv = C.moves[step];
+    int step, mv, N, conf, maxNum, startRes;
+    if (use_pre) {
+        step = prev_step + 1;
+        mv = L.misc[M_NX + 1];
+        N = ((mv >> 2) == prev_spi) ? L.misc[M_N] : L.misc[M_NX + 2];
+        conf = L.misc[M_NX + 3];
+        maxNum = L.misc[M_NX + 4];
+        startRes = L.misc[M_NX + 5];
+    } else {
+        step = S.step;
+        mv = C.moves[step];
+        const Species &F0 = C.sp[mv >> 2];
+        N = S.totalNum[mv >> 2];
+        conf = F0.conf;
+        maxNum = F0.maxNum;
+        startRes = F0.startRes;
+    }
     const int spi = mv >> 2, kind = mv & 3;
-    const Species &F = C.sp[spi];
-    const int N = S.totalNum[spi];
     int k;
-    if (kind == K_INS) k = (N >= F.maxNum) ? 0 : F.conf;
+    if (kind == K_INS) k = (N >= maxNum) ? 0 : conf;
     else if (kind == K_DEL) k = (N == 0) ? 0 : 1;
-    else k = F.conf < N ? F.conf : N;
+    else k = conf < N ? conf : N;
     if (k > C.maxk) k = C.maxk;
     if (kind != K_INS && k > 0) {
         // partial Fisher-Yates (gcmc_move_trn.h:135-149): pick i takes the value at
@@ -1354,7 +1422,7 @@ __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
             const float u[3] = {u01f(x.x), u01f(x.y), u01f(x.z)};
             for (int i = 0; i < 3; ++i) c[i] = C.grid[3 * g + i] + u[i] * C.dx;
         } else {
-            const float4 r = C.res_pos[F.startRes + C.tslot[t]];
+            const float4 r = C.res_pos[startRes + C.tslot[t]];
             c[0] = r.x; c[1] = r.y; c[2] = r.z;
             if (kind == K_TRN) {
                 const u32x4 x = draw(C.seed, (uint32_t)step, (uint32_t)t, P_JITTER);
@@ -1369,14 +1437,22 @@ __device__ inline void prep_step(const Rep &C, State &S, const Lds &L)
         }
         C.tc[t] = make_float4(c[0], c[1], c[2], 0.f);
     }
+    const int nitems = (kind == K_TRN || kind == K_ROT) ? 2 * k : k;
     if (tid == 0) {
         S.kind = kind;
         S.sp = spi;
         S.k = k;
-        S.nitems = (kind == K_TRN || kind == K_ROT) ? 2 * k : k;
+        S.nitems = nitems;
         S.done = 0;
     }
     __syncthreads();
+    return nitems;
+}
+
+template <int NT>
+__device__ inline int prep_step(const Rep &C, State &S, const Lds &L)
+{
+    return prep_step<NT>(C, S, L, false, 0, 0);
 }
 
 // XCD-local ordering.  Every workgroup touching a replica in the batched form
@@ -1391,18 +1467,20 @@ __device__ inline void xcd_acquire() { asm volatile("buffer_inv sc1" ::: "memory
 // claim-word priority of a replica's published move: lower is claimed first,
 // the replica with the most estimated work left first -- the run lasts as long
 // as its slowest chain
-__device__ inline int claim_prio(const Rep &C, const State &S)
+__device__ inline int claim_prio(const Rep &C, int step, int end)
 {
-    const uint32_t left = C.wcum[S.end] - C.wcum[S.step];
+    const uint32_t left = C.wcum[end] - C.wcum[step];
     return 0xFFFFFF - (int)(left < 0xFFFFFFu ? left : 0xFFFFFFu);
 }
 
 // advance past empty moves and publish the next move with work (batched form)
+// use_pre: the move after prev_step (species prev_spi) is in L.misc[M_NX..]
 template <int NT>
-__device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L)
+__device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L, bool use_pre,
+                                           int prev_step, int prev_spi)
 {
     while (true) {
-        if (S.step >= S.end) {
+        if (use_pre ? prev_step + 1 >= L.misc[M_NX + 0] : S.step >= S.end) {
             xcd_release();
             __syncthreads();
             if (threadIdx.x == 0) {
@@ -1411,8 +1489,18 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
             }
             break;
         }
-        prep_step<NT>(C, S, L);
-        if (S.nitems > 0) {
+        int step, end;
+        if (use_pre) {
+            step = prev_step + 1;
+            end = L.misc[M_NX + 0];
+        } else {
+            step = S.step;
+            end = S.end;
+        }
+        const int nitems = prep_step<NT>(C, S, L, use_pre, prev_step, prev_spi);
+        use_pre = false;
+        if (nitems > 0) {
+            const int prio = claim_prio(C, step, end);
             if (threadIdx.x == 0) {
                 S.t_pub = wall_clock64();
                 __hip_atomic_store(&S.done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1420,13 +1508,18 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
             xcd_release();
             __syncthreads();
             if (threadIdx.x == 0)
-                __hip_atomic_store(&S.claim, claim_word(claim_prio(C, S), S.nitems, 0), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&S.claim, claim_word(prio, nitems, 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
         accept_step<NT, false>(C, S, L);
     }
     __syncthreads();
+}
+
+template <int NT>
+__device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L)
+{
+    advance_and_publish<NT>(C, S, sch, L, false, 0, 0);
 }
 
 // ---- kernels ---------------------------------------------------------------------
@@ -1623,9 +1716,10 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             const unsigned long long tpub = S.t_pub;
             const int step0 = S.step;
             if (tid == 0) S.t_step += ta0 - tpub;
-            accept_step<NT, false>(C, S, L);
+            const MoveCtx mc = move_ctx(C, S);
+            accept_move<NT, false>(C, S, L, mc, true);
             const unsigned long long ta1 = wall_clock64();
-            advance_and_publish<NT>(C, S, sch, L);
+            advance_and_publish<NT>(C, S, sch, L, true, mc.step, mc.spi);
             if (tid == 0) {
                 const unsigned long long ta2 = wall_clock64();
                 S.t_accept += ta1 - ta0;

@@ -23,7 +23,7 @@ constexpr int MAXT = 32;             // species per system
 constexpr int MAXF = 20;             // atoms per fragment (AtomArray.atoms[20])
 constexpr int MAXK = 2048;           // trials per move (confBias)
 constexpr int ADJW = MAXK / 32;      // words of one adjacency row
-constexpr int MISC_INTS = 96;        // scratch ints of a workgroup (see engine_kernels.h)
+constexpr int MISC_INTS = 104;        // scratch ints of a workgroup (see engine_kernels.h)
 constexpr double BETA = 1.0 / (0.0083115 * 300.0);  // gcmc/cpp/gcmc.h:40-41
 constexpr double KCOUL = 1388.431112;                // gcmc/cpp/gcmc_energy.h:137
 constexpr float KCOUL_F = 1388.431112f;
