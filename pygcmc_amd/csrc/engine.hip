@@ -475,13 +475,12 @@ int compute_layout(gcmc_engine *E)
     E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
     auto up = [](int x) { return (x + 15) & ~15; };
     LdsLayout &l = E->lay;
-    l.inc_cap = 2 * E->nt;
+    l.inc_cap = INC_CAP;
     int o = 0;
     l.smob = o; o = up(o + 4 * (nst + 2));
     l.sfst = o; o = up(o + 4 * (nst + 2));
     l.sfix = o; o = up(o + 4 * (nst + 2));
     l.soff = o; o = up(o + 4 * (nst + 2));
-    l.incg = o; o = up(o + 4 * (l.inc_cap + 1));
     l.incs = o; o = up(o + 4 * (l.inc_cap + 1));
     l.incn = o; o = up(o + 4 * (l.inc_cap + 1));
     l.map_cap = 4096;

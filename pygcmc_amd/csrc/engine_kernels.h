@@ -43,8 +43,6 @@ __device__ __forceinline__ gptr<T> gbl(T *p)
 // scratch ints of a workgroup (Lds::misc)
 enum Misc : int {
     M_SCAN = 0,      // [0, NW]   scan partials, total at NW
-    M_WCNT = 20,     // [20, 36)  included groups per wave
-    M_NINC = 36,     //           included groups buffered
     M_NIN = 40,      // [40, 56)  per-wave included count
     M_HASH = 56,     // [56, 72)  per-wave set hash
     M_ACC = 80,      // accept: decision / count of accepted
@@ -58,7 +56,7 @@ enum Misc : int {
 };
 
 struct Lds {
-    int *smob, *sfst, *sfix, *soff, *incg, *incs, *incn;
+    int *smob, *sfst, *sfix, *soff, *incs, *incn;
     float4 *frag;
     int *frow;
     float4 *pair;
@@ -83,7 +81,6 @@ __device__ inline Lds carve(char *b, const LdsLayout &l)
     s.sfst = (int *)(b + l.sfst);
     s.sfix = (int *)(b + l.sfix);
     s.soff = (int *)(b + l.soff);
-    s.incg = (int *)(b + l.incg);
     s.incs = (int *)(b + l.incs);
     s.incn = (int *)(b + l.incn);
     s.frag = (float4 *)(b + l.frag);
@@ -501,14 +498,17 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     double acc64 = 0.0;  // FP64 pair terms (Coulomb)
     int nin = 0, natom = 0;
     uint32_t hs = 0;
-    if (tid == 0) L.misc[M_NINC] = 0;
-    __syncthreads();
-    for (int base = 0; base < M; base += NT) {
-        const int m = base + tid;
+    // Candidates in chunks of R rounds of NT.  Each wave appends the groups it
+    // includes to its own segment of SEG = 64 R slots, so no barrier is needed
+    // within a chunk; after a chunk the unused slots get zero atoms and one
+    // pair pass covers all segments (groups in segment order).
+    // candidate m: included (group test passed, not the excluded group) and
+    // its (first atom, atom count)
+    auto candidate = [&](int m, int2 &rr) -> bool {
         bool inc = false;
-        int gid = -1;
-        int2 rr = make_int2(0, 0);
+        rr = make_int2(0, 0);
         if (m < M) {
+            int gid;
             const int e = mapped ? (int)L.cand[m] : bsearch_le(L.soff, ne, m);
             const int local = m - L.soff[e];
             const int fc = L.sfix[e];
@@ -533,33 +533,30 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
                 hs += group_hash((uint32_t)gid);
             }
         }
-        const unsigned long long bal = __ballot(inc);
-        const int loff = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) L.misc[M_WCNT + wave] = __popcll(bal);
-        __syncthreads();
-        int wbase = L.misc[M_NINC];
-        for (int w = 0; w < wave; ++w) wbase += L.misc[M_WCNT + w];
-        if (inc) {
-            const int pos = wbase + loff;
-            L.incg[pos] = gid;
-            L.incs[pos] = rr.x;
-            L.incn[pos] = rr.y;
+        return inc;
+    };
+    // chunks of R rounds of NT; each wave appends to its own segment of
+    // SEG = 64 R slots, unused slots get zero atoms, and one workgroup
+    // pair pass covers all segments (groups in segment order)
+    constexpr int R = INC_CAP / NT, SEG = 64 * R;
+    for (int base = 0; base < M; base += R * NT) {
+        int cnt = 0;
+        for (int r = 0; r < R; ++r) {
+            int2 rr;
+            const bool inc = candidate(base + r * NT + tid, rr);
+            const unsigned long long bal = __ballot(inc);
+            if (inc) {
+                const int pos = wave * SEG + cnt + __popcll(bal & ((1ull << lane) - 1ull));
+                L.incs[pos] = rr.x;
+                L.incn[pos] = rr.y;
+            }
+            cnt += __popcll(bal);
         }
+        for (int q = cnt + lane; q < SEG; q += 64) L.incn[wave * SEG + q] = 0;
         __syncthreads();
-        if (tid == 0) {
-            int t = L.misc[M_NINC];
-            for (int w = 0; w < NW; ++w) t += L.misc[M_WCNT + w];
-            L.misc[M_NINC] = t;
-        }
-        __syncthreads();
-        const int ninc = L.misc[M_NINC];
-        if (ninc > L.inc_cap - NT || base + NT >= M) {
-            const unsigned long long tp0 = wall_clock64();
-            if (ninc > 0) natom += P.template run<NT, STAGED>(C, L, ninc, nf, acc, cmp, acc64);
-            tpair += wall_clock64() - tp0;
-            if (tid == 0) L.misc[M_NINC] = 0;
-            __syncthreads();
-        }
+        const unsigned long long tp0 = wall_clock64();
+        natom += P.template run<NT, STAGED>(C, L, INC_CAP, nf, acc, cmp, acc64);
+        tpair += wall_clock64() - tp0;
     }
     const double v = wave_scan_add((double)acc + (double)cmp + acc64);
     nin = wave_scan_add(nin);

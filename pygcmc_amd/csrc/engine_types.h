@@ -23,7 +23,8 @@ constexpr int MAXT = 32;             // species per system
 constexpr int MAXF = 20;             // atoms per fragment (AtomArray.atoms[20])
 constexpr int MAXK = 2048;           // trials per move (confBias)
 constexpr int ADJW = MAXK / 32;      // words of one adjacency row
-constexpr int MISC_INTS = 104;        // scratch ints of a workgroup (see engine_kernels.h)
+constexpr int MISC_INTS = 104;
+constexpr int INC_CAP = 1024;         // included-group slots per pair pass (a multiple of the workgroup size)        // scratch ints of a workgroup (see engine_kernels.h)
 constexpr double BETA = 1.0 / (0.0083115 * 300.0);  // gcmc/cpp/gcmc.h:40-41
 constexpr double KCOUL = 1388.431112;                // gcmc/cpp/gcmc_energy.h:137
 constexpr float KCOUL_F = 1388.431112f;
@@ -129,7 +130,7 @@ struct Sched {
 // byte offsets of the dynamic LDS carve-up (host computes, device applies)
 struct LdsLayout {
     int smob, sfst, sfix, soff;    // stencil entries (int[nst+2]): mobile-slot base, fixed start, fixed count, offset
-    int incg, incs, incn;          // included groups (int[inc_cap+1])
+    int incs, incn;                // included groups: first atom, atom count (int[inc_cap+1])
     int frag, frow;                // fragment pose (float4[MAXF], int[MAXF])
     int pair;                      // float4[nf*Y] (one species' rows) or -1
     int spt;                       // species table int4[MAXT] (startRes, maxNum, atomBase, nf)
