@@ -86,7 +86,10 @@ typedef struct gcmc_tuning {
     float cell_size;     /* target edge (A) of the group-centre cell list; 0 = 5 A */
     int persistent_wgs;  /* workgroups of the batched kernel; 0 = per CU by wg_size */
     int bucket_cap;      /* mobile groups per cell before overflow; 0 = auto */
-    int pair_lds;        /* 1 = stage the LJ pair table in LDS when it fits */
+    int pair_lds;        /* 1 = stage the moving species' LJ pair-table rows in LDS
+                            (copied with global_load_lds) when they fit; 0 (default) =
+                            read them from global memory, faster on the 64-replica
+                            benchmark; both give identical results */
     int wg_size;         /* threads per workgroup: 256, 512 or 1024; 0 = 256 */
     int fp32_coulomb;    /* 1 = Coulomb pair terms in FP32 like the LJ terms; 0 (default) =
                             Coulomb in FP64 from the exactly formed FP32 displacement */

@@ -410,6 +410,16 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     HIPX(put(&R.atype, B, atype.data(), atype.size()));
     HIPX(put((float4 **)&R.pair, B, pair.data(), pair.size()));
     HIPX(put((Species **)&R.sp, B, sp.data(), sp.size()));
+    {
+        // each species' rows of the pair table, atom by atom, contiguous
+        // (staging them in LDS then needs only the species index)
+        std::vector<float4> spair((size_t)T * MAXF * R.Y, make_float4(0.f, 0.f, 0.f, 0.f));
+        for (int s = 0; s < T; ++s)
+            for (int a = 0; a < sp[s].nf; ++a)
+                for (int y = 0; y < R.Y; ++y)
+                    spair[((size_t)s * MAXF + a) * R.Y + y] = pair[(size_t)sp[s].trow[a] + y];
+        HIPX(put((float4 **)&R.spair, B, spair.data(), spair.size()));
+    }
     HIPX(put((int **)&R.fix_start, B, fstart.data(), fstart.size()));
     HIPX(put((float4 **)&R.fix_rec, B, frec.data(), frec.size()));
     HIPX(put((int2 **)&R.fix_rng, B, frng.data(), frng.size()));
@@ -690,7 +700,6 @@ int gcmc_engine_create(gcmc_engine **out, const gcmc_tuning *tuning)
         return fail(GCMC_E_DEVICE, "no HIP device available");
     gcmc_engine *E = new gcmc_engine();
     if (tuning) E->tune = *tuning;
-    else E->tune.pair_lds = 1;
     hipError_t e = hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&E->ev0);
     if (e == hipSuccess) e = hipEventCreate(&E->ev1);
@@ -1331,7 +1340,6 @@ int runGCMC_hip(const InfoStruct *info, AtomArray *fragmentInfo, residue *residu
     if (!info) return fail(GCMC_E_ARG, "null info");
     gcmc_engine *E = nullptr;
     gcmc_tuning t{};
-    t.pair_lds = 1;
     int rc = gcmc_engine_create(&E, &t);
     if (rc) return rc;
     rc = gcmc_engine_add_system(E, info, fragmentInfo, residueInfo, atomInfo, grid, ff, moveArray,

@@ -428,15 +428,11 @@ __device__ inline void load_species_table(const Rep &C, const Lds &L)
 // pair-table rows of species spi's atoms into LDS, atom f's row at f * Y
 // (L.frow[f] = f * Y when staged)
 template <int NT>
-__device__ inline void stage_species_rows(const Rep &C, int spi, const Lds &L)
+__device__ inline void stage_species_rows(const Rep &C, int spi, int nf, const Lds &L)
 {
-    const auto &F = gbl(C.sp)[spi];
-    const auto GPT = gbl(C.pair);
-    const int Y = C.Y, n = F.nf * Y;
-    for (int i = threadIdx.x; i < n; i += NT) {
-        const int f = i / Y;
-        L.pair[i] = GPT[F.trow[f] + (i - f * Y)];
-    }
+    const auto SP = gbl(C.spair) + (size_t)spi * MAXF * C.Y;
+    const int n = nf * C.Y;
+    for (int i = threadIdx.x; i < n; i += NT) L.pair[i] = SP[i];
 }
 
 // ---- calcEnergy over the cell list -----------------------------------------
@@ -553,6 +549,9 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
             cnt += __popcll(bal);
         }
         for (int q = cnt + lane; q < SEG; q += 64) L.incn[wave * SEG + q] = 0;
+        // staged pair rows were loaded into LDS asynchronously (eval_item):
+        // every wave's copies have landed once the barrier passes
+        if (STAGED) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         const unsigned long long tp0 = wall_clock64();
         natom += P.template run<NT, STAGED>(C, L, INC_CAP, nf, acc, cmp, acc64);
@@ -710,7 +709,17 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
     // items of one replica (stage_rep = false), the species' pair rows while
     // they are also of one species (stage_rows = false)
     if (stage_rep) load_species_table(C, L);
-    if (STAGED && stage_rows) stage_species_rows<NT>(C, spi, L);
+    // the species' pair rows go from global memory straight into LDS
+    // (global_load_lds: no registers, no wait here); the pose, stencil and
+    // candidate work runs meanwhile, and the wait is before the first pair
+    // pass (eval_energy_pass)
+    const bool stg = STAGED && stage_rows;
+    if (stg) {
+        const int nrow = nf * C.Y;
+        const auto SP = gbl(C.spair) + (size_t)spi * MAXF * C.Y;
+        for (int b = wave * 64; b < nrow; b += NT)
+            if (b + lane < nrow) __builtin_amdgcn_global_load_lds((const void *)(SP + b + lane), (void *)&L.pair[b], 16, 0, 0);
+    }
     const float4 tcv = tc[t];
     const int slot = (kind == K_INS) ? -1 : gbl(C.tslot)[t];
     const int gid = (kind == K_INS) ? -1 : F.startRes + slot;
@@ -1754,7 +1763,7 @@ __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int s
     const Rep &C = reps[r];
     const Species &F = C.sp[spi];
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x;
-    if (STAGED) stage_species_rows<NT>(C, spi, L);
+    if (STAGED) stage_species_rows<NT>(C, spi, nf, L);
     load_species_table(C, L);
     if (tid < nf) {
         const float *p = poses + ((size_t)i * nf + tid) * 3;

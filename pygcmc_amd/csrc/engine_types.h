@@ -74,6 +74,7 @@ struct Rep {
     float4 *atom;
     int *atype;
     const float4 *pair;      // (100 sigma^2 hi, 4 eps, 100 sigma^2 lo, 0) per (fragment type, type)
+    const float4 *spair;     // [T][MAXF][Y]: pair rows of each species' atoms, contiguous
     const Species *sp;
     const int *fix_start;
     const float4 *fix_rec;   // (x, y, z, gid bits)

@@ -137,7 +137,7 @@ def host_stencil_covers(centres, groups, cryst, startxyz, cutoff, cell_size=0.0)
 class Engine:
     """Independent GCMC systems on one GPU (see module docstring)."""
 
-    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=1, wg_size=0, fp32_coulomb=0):
+    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=0, wg_size=0, fp32_coulomb=0):
         self._h = ctypes.c_void_p()
         t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds, wg_size, fp32_coulomb)
         _check(lib().gcmc_engine_create(ctypes.byref(self._h), ctypes.byref(t)))

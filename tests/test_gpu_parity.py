@@ -157,6 +157,29 @@ def test_plain_batched_graph_identical(wg_size):
         assert st['energy'] == outs[0][2]['energy']
 
 
+def test_pair_rows_in_lds_identical():
+    """Pair-table rows staged in LDS (copied with global_load_lds) or read
+    from global memory: the same values, so bit-identical energies and
+    chains, in the batched and plain forms."""
+    s = systems.multi(nsteps=300, seed=13)
+    outs = []
+    for pair_lds in (0, 1):
+        for mode in ('batched', 'plain'):
+            g = systems.copy(s)
+            eng = Engine(pair_lds=pair_lds)
+            eng.add_system(g)
+            eng.upload()
+            eng.set_trace(0, 300, int(s['frag']['confBias'].max()))
+            eng.run(300, mode)
+            tr = eng.get_trace(0)
+            eng.download(0)
+            eng.close()
+            outs.append((tr, g))
+    for tr, g in outs[1:]:
+        assert np.array_equal(tr['e'].view(np.uint64), outs[0][0]['e'].view(np.uint64))
+        assert np.array_equal(g['atoms'].view(np.uint8), outs[0][1]['atoms'].view(np.uint8))
+
+
 def test_replicas_match_single_runs():
     """Uneven replicas in one persistent kernel give the same chains as
     each system run alone."""
