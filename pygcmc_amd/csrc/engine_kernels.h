@@ -312,14 +312,20 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
     double accd = 0.0;
     // one environment atom against every fragment atom; a thread takes atoms
     // t, t + NT, t + 2 NT, ... in that order (the order of its sums)
+    auto pair_row = [&](const int f, const int ty) {
+        if constexpr (STAGED)
+            return L.pair[L.frow[f] + ty];
+        else
+            return GPT[L.frow[f] + ty];
+    };
     auto atom_terms = [&](const float4 at, const int ty) {
+        // the table entry of fragment atom f + 1 is fetched while atom f's
+        // terms are computed (the last fetch repeats atom nf - 1's)
+        float4 pn = pair_row(0, ty);
         for (int f = 0; f < nf; ++f) {
             const float4 fr = L.frag[f];
-            float4 pp;
-            if constexpr (STAGED)
-                pp = L.pair[L.frow[f] + ty];
-            else
-                pp = GPT[L.frow[f] + ty];
+            const float4 pp = pn;
+            pn = pair_row(min(f + 1, nf - 1), ty);
             // displacement as an exact sum s + e (TwoSum); the image shift
             // s - n L is exact for |n| <= 1 (Sterbenz)
             float ex, ey, ez;
