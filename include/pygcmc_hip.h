@@ -240,6 +240,25 @@ int gcmc_engine_ewald_potential(gcmc_engine *e, int replica, int n, const float 
 int gcmc_cavity_grid(int natom, const double *xyz, const double *startxyz, double grid_dx,
                      const int *grid_n, int max_points, float *grid, int *npoints);
 
+/* ---- cavity grid in the reference's order (gcmc/python/dataset.py:66-109) -
+ * The free cells of the grid_n[0] x grid_n[1] x grid_n[2] grid in the
+ * iteration order of the CPython (3.8+) set the reference builds with
+ * {(x, y, z) for x in range(n0) for y in range(n1) for z in range(n2)} and
+ * discards the occupied cells from.  occupied[n0*n1*n2] (x slowest, z
+ * fastest; nonzero = occupied) or NULL for none.  Writes the free-cell count
+ * to *count and, when cells is not NULL and the count is at most max_cells,
+ * the cells' (x, y, z) indices to cells[3*count].  At most 2^28 cells.  Host
+ * code, no device needed. */
+int gcmc_grid_set_order(const int *grid_n, const unsigned char *occupied, long long max_cells, int *cells,
+                        long long *count);
+
+/* ---- fixed-atom grouping (gcmc/python/dataset.py:175-188) ----------------
+ * group[i] for the n atoms xyz[n][3] (FP64, in order): an atom opens a new
+ * group when sqrt(dx**2 + dy**2 + dz**2) > cutoff from the first atom of the
+ * current group (squares as Python's float power), groups numbered from 0.
+ * Host code, no device needed. */
+int gcmc_fixed_groups(int n, const double *xyz, double cutoff, int *group);
+
 /* ---- smooth particle-mesh Ewald, reciprocal part ---------------------------
  * No counterpart in the reference (its PME flag, gcmc/cpp/gcmc.h:91, is never
  * read).  Orthorhombic box box[3] (A), grid[3] points (4..4096 each), Ewald

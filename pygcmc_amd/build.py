@@ -1,7 +1,7 @@
 """Build the native parts of pygcmc_amd in-tree.
 
 * ``libpygcmc_hip.so`` -- HIP engine + C ABI (``include/pygcmc_hip.h``),
-  hipcc for gfx950.
+  hipcc for gfx950, plus host-only set-up code (g++).
 * ``gpu.cpython-*.so`` -- CPython module ``pygcmc_amd.gpu`` exposing
   ``runGCMC(info, fragmentInfo, residueInfo, atomInfo, grid, ff, moveArray)``
   like pyGCMC's ``gcmc.gpu`` (``gcmc/cpp/gcmc.cpp:162-225``), linked against
@@ -25,6 +25,8 @@ HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('PYGCMC_ARCH', 'gfx950')
 
 HIP_SOURCES = ['engine.hip', 'grid.hip', 'pme.hip']
+# host-only C++ (system set-up), compiled with the host compiler
+HOST_SOURCES = ['setup_host.cpp']
 
 HIP_FLAGS = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
@@ -47,6 +49,11 @@ def build_library():
         obj = os.path.join(HERE, '_build', os.path.splitext(src)[0] + '.o')
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         _run([HIPCC, *HIP_FLAGS, '-I', os.path.join(ROOT, 'include'), '-c',
+              os.path.join(CSRC, src), '-o', obj])
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(HERE, '_build', os.path.splitext(src)[0] + '.o')
+        _run(['g++', '-O2', '-std=c++17', '-fPIC', '-Wall', '-I', os.path.join(ROOT, 'include'), '-c',
               os.path.join(CSRC, src), '-o', obj])
         objs.append(obj)
     rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), 'lib')

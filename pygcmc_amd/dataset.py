@@ -5,8 +5,9 @@ Each function restates one piece of pyGCMC's Python set-up with the same
 arithmetic (FP64 from coordinates, then FP32 in the packed arrays):
 
 * ``get_move``      -- ``gcmc/python/dataset.py:141-171``
-* ``get_grid``      -- ``gcmc/python/dataset.py:21-139``
-* ``set_fixcut``    -- ``gcmc/python/dataset.py:175-188``
+* ``get_grid``      -- ``gcmc/python/dataset.py:21-139`` (the reference's
+  set order native)
+* ``set_fixcut``    -- ``gcmc/python/dataset.py:175-188`` (native)
 * ``ff_pair_table`` -- ``gcmc/python/simulation.py:37-73``
 * ``pack_system``   -- ``gcmc/python/dataset.py:190-344``
 
@@ -14,7 +15,7 @@ The packed arrays are exactly what ``gcmc.gpu.runGCMC`` receives in the
 reference (``gcmc/python/gcmc.py:41-49``).
 """
 
-import math
+import ctypes
 import random as _random
 
 import numpy as np
@@ -90,15 +91,7 @@ def get_grid(xyz, cryst, grid_dx=1.0, cavity_bias=True, order='linear'):
         free_lin = np.arange(total, dtype=np.int64)
         cavity_factor = 1.0
     if order == 'cpython':
-        cells = {(x, y, z) for x in range(n[0]) for y in range(n[1]) for z in range(n[2])}
-        if cavity_bias:
-            occ_lin = np.flatnonzero(occ)
-            for c in occ_lin.tolist():
-                z = c % n[2]
-                y = (c // n[2]) % n[1]
-                x = c // (n[1] * n[2])
-                cells.discard((x, y, z))
-        coords = np.array(list(cells), dtype=np.int64).reshape(-1, 3)
+        coords = grid_set_order(n, occ if cavity_bias else None).astype(np.int64)
     elif order == 'linear':
         coords = np.stack([free_lin // (n[1] * n[2]), (free_lin // n[2]) % n[1],
                            free_lin % n[2]], axis=1)
@@ -113,17 +106,41 @@ def get_grid(xyz, cryst, grid_dx=1.0, cavity_bias=True, order='linear'):
 def set_fixcut(xyz_fixed, fix_cutoff=6.0):
     """Consecutive grouping of fixed atoms (``gcmc/python/dataset.py:175-188``):
     an atom opens a new group when it lies more than ``fix_cutoff`` from the
-    first atom of the current group (no periodic images).  Returns int group
+    first atom of the current group (no periodic images; squares as Python's
+    float ``** 2``).  Native (``gcmc_fixed_groups``).  Returns int group
     index per atom."""
-    seq = np.empty(len(xyz_fixed), dtype=np.int64)
-    x0 = y0 = z0 = float('inf')
-    n = -1
-    for i, (x, y, z) in enumerate(np.asarray(xyz_fixed, dtype=np.float64).tolist()):
-        if math.sqrt((x - x0) ** 2 + (y - y0) ** 2 + (z - z0) ** 2) > fix_cutoff:
-            n += 1
-            x0, y0, z0 = x, y, z
-        seq[i] = n
-    return seq
+    from . import engine as _engine
+    xyz = np.ascontiguousarray(xyz_fixed, dtype=np.float64).reshape(-1, 3)
+    out = np.empty(len(xyz), dtype=np.int32)
+    L = _engine.lib()
+    L.gcmc_fixed_groups.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p]
+    _engine._check(L.gcmc_fixed_groups(len(xyz), xyz.ctypes.data, float(fix_cutoff), out.ctypes.data))
+    return out.astype(np.int64)
+
+
+def grid_set_order(n, occupied=None):
+    """Free cells ``int32[G, 3]`` of an ``n[0] x n[1] x n[2]`` grid in the
+    iteration order of the CPython set the reference builds and discards the
+    occupied cells from (``gcmc/python/dataset.py:66-109``); ``occupied``:
+    bool[n0*n1*n2], x slowest, or None.  Native (``gcmc_grid_set_order``):
+    the set's slot order from CPython's tuple hash and table rules, without
+    Python objects."""
+    from . import engine as _engine
+    L = _engine.lib()
+    P = ctypes.c_void_p
+    L.gcmc_grid_set_order.argtypes = [P, P, ctypes.c_longlong, P, ctypes.POINTER(ctypes.c_longlong)]
+    nn = np.ascontiguousarray(n, dtype=np.int32)
+    occ = None
+    if occupied is not None:
+        occ = np.ascontiguousarray(occupied, dtype=np.uint8).ravel()
+        if occ.size != int(nn[0]) * int(nn[1]) * int(nn[2]):
+            raise ValueError('occupied must hold n0*n1*n2 cells')
+    cnt = ctypes.c_longlong()
+    occ_p = None if occ is None else occ.ctypes.data
+    _engine._check(L.gcmc_grid_set_order(nn.ctypes.data, occ_p, 0, None, ctypes.byref(cnt)))
+    cells = np.empty((cnt.value, 3), dtype=np.int32)
+    _engine._check(L.gcmc_grid_set_order(nn.ctypes.data, occ_p, cnt.value, cells.ctypes.data, ctypes.byref(cnt)))
+    return cells
 
 
 def ff_pair_table(types1, types2, nb, nbfix=None):
