@@ -167,6 +167,8 @@ int check_system(const InfoStruct *info, const AtomArray *frag, int nmoves)
     if (info->cryst[0] <= 0 || info->cryst[1] <= 0 || info->cryst[2] <= 0)
         return fail(GCMC_E_ARG, "box edges must be positive");
     if (info->totalGridNum < 1) return fail(GCMC_E_ARG, "empty cavity grid");
+    if (info->totalResNum < 0 || info->totalResNum > GID_MASK)
+        return fail(GCMC_E_ARG, "totalResNum must be in [0, 2^24)");
     if (nmoves < 0) return fail(GCMC_E_ARG, "negative move count");
     for (int s = 0; s < info->fragTypeNum; ++s) {
         const AtomArray &F = frag[s];
@@ -393,8 +395,9 @@ int upload_rep(gcmc_engine *E, HostRep &H)
             else pos = R.ovf_base + mcnt[R.ncell]++;
             mslot[pos] = g;
             mloc[g] = pos;
+            const int bits = mob_word(g, s);
             float w;
-            std::memcpy(&w, &g, 4);
+            std::memcpy(&w, &bits, 4);
             mrec[pos] = make_float4(r.position[0], r.position[1], r.position[2], w);
         }
     }

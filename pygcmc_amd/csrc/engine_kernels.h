@@ -500,52 +500,61 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     double acc64 = 0.0;  // FP64 pair terms (Coulomb)
     int nin = 0, natom = 0;
     uint32_t hs = 0;
-    // Candidates in chunks of R rounds of NT.  Each wave appends the groups it
-    // includes to its own segment of SEG = 64 R slots, so no barrier is needed
-    // within a chunk; after a chunk the unused slots get zero atoms and one
-    // pair pass covers all segments (groups in segment order).
-    // candidate m: included (group test passed, not the excluded group) and
-    // its (first atom, atom count)
-    auto candidate = [&](int m, int2 &rr) -> bool {
-        bool inc = false;
-        rr = make_int2(0, 0);
-        if (m < M) {
-            int gid;
-            const int e = mapped ? (int)L.cand[m] : bsearch_le(L.soff, ne, m);
-            const int local = m - L.soff[e];
-            const int fc = L.sfix[e];
-            float4 p;
-            if (local < fc) {
-                const int q = L.sfst[e] + local;
-                p = fix_rec[q];
-                rr = fix_rng[q];
-                gid = __float_as_int(p.w);
-            } else {
-                const int idx = L.smob[e] + (local - fc);
-                p = mob_rec[idx];
-                gid = __float_as_int(p.w);
-                for (int s = 0; s < C.T; ++s) {
-                    const int4 sp = L.spt[s];
-                    if (gid >= sp.x && gid < sp.x + sp.y) rr = make_int2(sp.z + (gid - sp.x) * sp.w, sp.w);
-                }
-            }
-            inc = (gid != exclude) && within(cx, cy, cz, p.x, p.y, p.z, C.ax, C.r2max);
-            if (inc) {
-                nin += 1;
-                hs += group_hash((uint32_t)gid);
-            }
-        }
-        return inc;
-    };
-    // chunks of R rounds of NT; each wave appends to its own segment of
-    // SEG = 64 R slots, unused slots get zero atoms, and one workgroup
-    // pair pass covers all segments (groups in segment order)
+    // Candidates in chunks of R rounds of NT.  A thread first issues the
+    // record loads of its R candidates of the chunk (one memory round trip
+    // for all of them), then tests them in order; each wave appends the
+    // groups it includes to its own segment of SEG = 64 R slots, so no
+    // barrier is needed within a chunk; after a chunk the unused slots get
+    // zero atoms and one pair pass covers all segments (groups in segment
+    // order).
     constexpr int R = INC_CAP / NT, SEG = 64 * R;
     for (int base = 0; base < M; base += R * NT) {
-        int cnt = 0;
+        float4 rec[R];
+        int2 rng[R];
+        bool fixed[R];
+#pragma unroll
         for (int r = 0; r < R; ++r) {
-            int2 rr;
-            const bool inc = candidate(base + r * NT + tid, rr);
+            const int m = base + r * NT + tid;
+            rec[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            rng[r] = make_int2(0, 0);
+            fixed[r] = false;
+            if (m < M) {
+                const int e = mapped ? (int)L.cand[m] : bsearch_le(L.soff, ne, m);
+                const int local = m - L.soff[e];
+                const int fc = L.sfix[e];
+                if (local < fc) {
+                    const int q = L.sfst[e] + local;
+                    rec[r] = fix_rec[q];
+                    rng[r] = fix_rng[q];
+                    fixed[r] = true;
+                } else {
+                    rec[r] = mob_rec[L.smob[e] + (local - fc)];
+                }
+            }
+        }
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int m = base + r * NT + tid;
+            bool inc = false;
+            int2 rr = rng[r];
+            if (m < M) {
+                const float4 p = rec[r];
+                int gid;
+                if (fixed[r]) {
+                    gid = __float_as_int(p.w);
+                } else {
+                    const int word = __float_as_int(p.w);
+                    gid = word & GID_MASK;
+                    const int4 sp = L.spt[word >> GID_BITS];   // (startRes, maxNum, atomBase, nf)
+                    rr = make_int2(sp.z + (gid - sp.x) * sp.w, sp.w);
+                }
+                inc = (gid != exclude) && within(cx, cy, cz, p.x, p.y, p.z, C.ax, C.r2max);
+                if (inc) {
+                    nin += 1;
+                    hs += group_hash((uint32_t)gid);
+                }
+            }
             const unsigned long long bal = __ballot(inc);
             if (inc) {
                 const int pos = wave * SEG + cnt + __popcll(bal & ((1ull << lane) - 1ull));
@@ -847,7 +856,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
 }
 
 // ---- cell-list maintenance (single thread) ----------------------------------
-__device__ inline void cell_insert(const Rep &C, int gid, float x, float y, float z)
+__device__ inline void cell_insert(const Rep &C, int gid, int spi, float x, float y, float z)
 {
     const int c = cell_index(C.L, C.start, C.inv_h, C.nc, x, y, z);
     const int n = C.mob_cnt[c];
@@ -861,7 +870,7 @@ __device__ inline void cell_insert(const Rep &C, int gid, float x, float y, floa
         C.mob_cnt[C.ncell] = o + 1;
     }
     C.mob_slot[pos] = gid;
-    C.mob_rec[pos] = make_float4(x, y, z, __int_as_float(gid));
+    C.mob_rec[pos] = make_float4(x, y, z, __int_as_float(mob_word(gid, spi)));
     C.mob_loc[gid] = pos;
 }
 
@@ -885,13 +894,13 @@ __device__ inline void cell_remove(const Rep &C, int gid)
     C.mob_loc[gid] = -1;
 }
 
-// group `from` now lives in slot `to` (its centre is unchanged)
+// group `from` now lives in slot `to` (its centre and species are unchanged)
 __device__ inline void cell_rename(const Rep &C, int from, int to)
 {
     const int pos = C.mob_loc[from];
     C.mob_slot[pos] = to;
     float4 r = C.mob_rec[pos];
-    r.w = __int_as_float(to);
+    r.w = __int_as_float((__float_as_int(r.w) & ~GID_MASK) | to);
     C.mob_rec[pos] = r;
     C.mob_loc[to] = pos;
     C.mob_loc[from] = -1;
@@ -1215,13 +1224,13 @@ __device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int
             const float4 c = C.tc[ts.x];
             if (kind == K_INS) {
                 C.res_pos[gid] = make_float4(c.x, c.y, c.z, 0.f);
-                cell_insert(C, gid, c.x, c.y, c.z);
+                cell_insert(C, gid, spi, c.x, c.y, c.z);
                 if (trace && ts.x < C.tr.max_k) C.tr.idx[row * C.tr.max_k + ts.x] = ts.y;
             } else if (kind == K_TRN) {
                 const float4 old = C.res_pos[gid];
                 C.res_pos[gid] = make_float4(c.x, c.y, c.z, old.w);
                 cell_remove(C, gid);
-                cell_insert(C, gid, c.x, c.y, c.z);
+                cell_insert(C, gid, spi, c.x, c.y, c.z);
             }
         }
         if (kind == K_INS) S.totalNum[spi] = L.misc[M_N];

@@ -25,6 +25,10 @@ constexpr int MAXK = 2048;           // trials per move (confBias)
 constexpr int ADJW = MAXK / 32;      // words of one adjacency row
 constexpr int MISC_INTS = 104;
 constexpr int INC_CAP = 1024;         // included-group slots per pair pass (a multiple of the workgroup size)        // scratch ints of a workgroup (see engine_kernels.h)
+// a mobile group's cell record carries its group id and species in one word
+constexpr int GID_BITS = 24;         // group ids below 2^24
+constexpr int GID_MASK = (1 << GID_BITS) - 1;
+__host__ __device__ inline int mob_word(int gid, int spi) { return gid | (spi << GID_BITS); }
 constexpr double BETA = 1.0 / (0.0083115 * 300.0);  // gcmc/cpp/gcmc.h:40-41
 constexpr double KCOUL = 1388.431112;                // gcmc/cpp/gcmc_energy.h:137
 constexpr float KCOUL_F = 1388.431112f;
@@ -80,7 +84,7 @@ struct Rep {
     const float4 *fix_rec;   // (x, y, z, gid bits)
     const int2 *fix_rng;     // (atomStart, atomNum), parallel to fix_rec
     int *mob_cnt, *mob_slot, *mob_loc;
-    float4 *mob_rec;         // bucket entries (x, y, z, gid bits), parallel to mob_slot
+    float4 *mob_rec;         // bucket entries (x, y, z, mob_word(gid, species) bits), parallel to mob_slot
     const int *stencil;      // packed (dx+128) | (dy+128)<<8 | (dz+128)<<16
     // per-step buffers
     float4 *tc;              // trial centres
