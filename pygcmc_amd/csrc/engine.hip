@@ -423,11 +423,14 @@ int upload_rep(gcmc_engine *E, HostRep &H)
                     spair[((size_t)s * MAXF + a) * R.Y + y] = pair[(size_t)sp[s].trow[a] + y];
         HIPX(put((float4 **)&R.spair, B, spair.data(), spair.size()));
     }
-    HIPX(put((int **)&R.fix_start, B, fstart.data(), fstart.size()));
+    {
+        std::vector<int2> cellv(R.ncell + 1);
+        for (int c = 0; c <= R.ncell; ++c) cellv[c] = make_int2(fstart[c], mcnt[c]);
+        HIPX(put(&R.cell, B, cellv.data(), cellv.size()));
+    }
     HIPX(put((float4 **)&R.fix_rec, B, frec.data(), frec.size()));
     HIPX(put((int2 **)&R.fix_rng, B, frng.data(), frng.size()));
     HIPX(put(&R.mob_rec, B, mrec.data(), mrec.size()));
-    HIPX(put(&R.mob_cnt, B, mcnt.data(), mcnt.size()));
     HIPX(put(&R.mob_slot, B, mslot.data(), mslot.size()));
     HIPX(put(&R.mob_loc, B, mloc.data(), mloc.size()));
     HIPX(put((int **)&R.stencil, B, st.data(), st.size()));

@@ -456,8 +456,7 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const auto stencil = gbl(C.stencil);
-    const auto fix_start = gbl(C.fix_start);
-    const auto mob_cnt = gbl(C.mob_cnt);
+    const auto cellv = gbl(C.cell);
     const auto fix_rec = gbl(C.fix_rec);
     const auto fix_rng = gbl(C.fix_rng);
     const auto mob_rec = gbl(C.mob_rec);
@@ -478,14 +477,15 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
             i1 += i1 < 0 ? C.nc[1] : (i1 >= C.nc[1] ? -C.nc[1] : 0);
             i2 += i2 < 0 ? C.nc[2] : (i2 >= C.nc[2] ? -C.nc[2] : 0);
             cell = (i0 * C.nc[1] + i1) * C.nc[2] + i2;
-            fs = fix_start[cell];
-            fc = fix_start[cell + 1] - fs;
-            mc = mob_cnt[cell];
+            const int2 ci = cellv[cell];
+            fs = ci.x;
+            fc = cellv[cell + 1].x - fs;
+            mc = ci.y;
             L.smob[e] = cell * C.cap;
         } else {
             fs = 0;
             fc = 0;
-            mc = mob_cnt[C.ncell];
+            mc = cellv[C.ncell].y;
             L.smob[e] = C.ovf_base;
         }
         L.sfst[e] = fs;
@@ -859,15 +859,15 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
 __device__ inline void cell_insert(const Rep &C, int gid, int spi, float x, float y, float z)
 {
     const int c = cell_index(C.L, C.start, C.inv_h, C.nc, x, y, z);
-    const int n = C.mob_cnt[c];
+    const int n = C.cell[c].y;
     int pos;
     if (n < C.cap) {
         pos = c * C.cap + n;
-        C.mob_cnt[c] = n + 1;
+        C.cell[c].y = n + 1;
     } else {
-        const int o = C.mob_cnt[C.ncell];
+        const int o = C.cell[C.ncell].y;
         pos = C.ovf_base + o;
-        C.mob_cnt[C.ncell] = o + 1;
+        C.cell[C.ncell].y = o + 1;
     }
     C.mob_slot[pos] = gid;
     C.mob_rec[pos] = make_float4(x, y, z, __int_as_float(mob_word(gid, spi)));
@@ -885,12 +885,12 @@ __device__ inline void cell_remove(const Rep &C, int gid)
         c = pos / C.cap;
         base = c * C.cap;
     }
-    const int n = C.mob_cnt[c] - 1;
+    const int n = C.cell[c].y - 1;
     const int moved = C.mob_slot[base + n];
     C.mob_slot[pos] = moved;
     C.mob_rec[pos] = C.mob_rec[base + n];
     C.mob_loc[moved] = pos;
-    C.mob_cnt[c] = n;
+    C.cell[c].y = n;
     C.mob_loc[gid] = -1;
 }
 

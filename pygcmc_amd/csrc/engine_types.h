@@ -8,7 +8,7 @@
 //   atype    int[A]      LJ type (column of the pair table)
 //   pair     float2[X*Y] (100*sigma^2, 4*epsilon) per (fragment type, type)
 // plus a cell list over group centres: fixed groups in a static CSR
-// (fix_start / fix_rec sorted by cell), mobile groups (fragment slots) in
+// (cell / fix_rec sorted by cell), mobile groups (fragment slots) in
 // per-cell buckets of `cap` slots with one overflow bucket.
 #pragma once
 
@@ -80,10 +80,13 @@ struct Rep {
     const float4 *pair;      // (100 sigma^2 hi, 4 eps, 100 sigma^2 lo, 0) per (fragment type, type)
     const float4 *spair;     // [T][MAXF][Y]: pair rows of each species' atoms, contiguous
     const Species *sp;
-    const int *fix_start;
+    // per cell c in [0, ncell]: (first fixed record, mobile count); the fixed
+    // records of cell c are [cell[c].x, cell[c + 1].x), and cell[ncell].y
+    // counts the overflow bucket
+    int2 *cell;
     const float4 *fix_rec;   // (x, y, z, gid bits)
     const int2 *fix_rng;     // (atomStart, atomNum), parallel to fix_rec
-    int *mob_cnt, *mob_slot, *mob_loc;
+    int *mob_slot, *mob_loc;
     float4 *mob_rec;         // bucket entries (x, y, z, mob_word(gid, species) bits), parallel to mob_slot
     const int *stencil;      // packed (dx+128) | (dy+128)<<8 | (dz+128)<<16
     // per-step buffers
