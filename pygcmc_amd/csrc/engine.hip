@@ -365,8 +365,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
         fstart[fcell[g] + 1] += 1;
     }
     for (int c = 0; c < R.ncell; ++c) fstart[c + 1] += fstart[c];
-    std::vector<float4> frec(std::max(R.nfix, 1));
-    std::vector<int2> frng(std::max(R.nfix, 1));
+    std::vector<FixRec> frec(std::max(R.nfix, 1));
     {
         std::vector<int> fill(fstart.begin(), fstart.end() - 1);
         for (int g = 0; g < R.nfix; ++g) {
@@ -375,8 +374,9 @@ int upload_rep(gcmc_engine *E, HostRep &H)
             float w;
             std::memcpy(&w, &bits, 4);
             const int q = fill[fcell[g]]++;
-            frec[q] = make_float4(r.position[0], r.position[1], r.position[2], w);
-            frng[q] = make_int2(r.atomStart, r.atomNum);
+            frec[q].p = make_float4(r.position[0], r.position[1], r.position[2], w);
+            frec[q].rng = make_int2(r.atomStart, r.atomNum);
+            frec[q].pad = make_int2(0, 0);
         }
     }
     // mobile groups: buckets + overflow
@@ -428,8 +428,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
         for (int c = 0; c <= R.ncell; ++c) cellv[c] = make_int2(fstart[c], mcnt[c]);
         HIPX(put(&R.cell, B, cellv.data(), cellv.size()));
     }
-    HIPX(put((float4 **)&R.fix_rec, B, frec.data(), frec.size()));
-    HIPX(put((int2 **)&R.fix_rng, B, frng.data(), frng.size()));
+    HIPX(put((FixRec **)&R.fix, B, frec.data(), frec.size()));
     HIPX(put(&R.mob_rec, B, mrec.data(), mrec.size()));
     HIPX(put(&R.mob_slot, B, mslot.data(), mslot.size()));
     HIPX(put(&R.mob_loc, B, mloc.data(), mloc.size()));

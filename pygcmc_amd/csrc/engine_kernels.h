@@ -457,8 +457,7 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const auto stencil = gbl(C.stencil);
     const auto cellv = gbl(C.cell);
-    const auto fix_rec = gbl(C.fix_rec);
-    const auto fix_rng = gbl(C.fix_rng);
+    const auto fix = gbl(C.fix);
     const auto mob_rec = gbl(C.mob_rec);
     const int cc = cell_index(C.L, C.start, C.inv_h, C.nc, cx, cy, cz);
     const int c2 = cc % C.nc[2];
@@ -524,8 +523,8 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
                 const int fc = L.sfix[e];
                 if (local < fc) {
                     const int q = L.sfst[e] + local;
-                    rec[r] = fix_rec[q];
-                    rng[r] = fix_rng[q];
+                    rec[r] = fix[q].p;
+                    rng[r] = fix[q].rng;
                     fixed[r] = true;
                 } else {
                     rec[r] = mob_rec[L.smob[e] + (local - fc)];

@@ -8,7 +8,7 @@
 //   atype    int[A]      LJ type (column of the pair table)
 //   pair     float2[X*Y] (100*sigma^2, 4*epsilon) per (fragment type, type)
 // plus a cell list over group centres: fixed groups in a static CSR
-// (cell / fix_rec sorted by cell), mobile groups (fragment slots) in
+// (cell / fix sorted by cell), mobile groups (fragment slots) in
 // per-cell buckets of `cap` slots with one overflow bucket.
 #pragma once
 
@@ -58,6 +58,15 @@ struct DevTrace {
     unsigned long long *tick;  // [S][4] publish, accept start, accept end, prep end (batched form)
 };
 
+// a fixed group of the cell list: centre and id, atom range (one 32 B
+// record, so a candidate's two loads fall in one cache line)
+struct __align__(16) FixRec {
+    float4 p;                // (x, y, z, gid bits)
+    int2 rng;                // (atomStart, atomNum)
+    int2 pad;
+};
+static_assert(sizeof(FixRec) == 32, "FixRec layout");
+
 struct Rep {
     AxisCut ax[3];
     float L[3], start[3], invL[3];
@@ -84,8 +93,7 @@ struct Rep {
     // records of cell c are [cell[c].x, cell[c + 1].x), and cell[ncell].y
     // counts the overflow bucket
     int2 *cell;
-    const float4 *fix_rec;   // (x, y, z, gid bits)
-    const int2 *fix_rng;     // (atomStart, atomNum), parallel to fix_rec
+    const FixRec *fix;       // fixed groups sorted by cell
     int *mob_slot, *mob_loc;
     float4 *mob_rec;         // bucket entries (x, y, z, mob_word(gid, species) bits), parallel to mob_slot
     const int *stencil;      // packed (dx+128) | (dy+128)<<8 | (dz+128)<<16
