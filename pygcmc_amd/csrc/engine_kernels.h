@@ -1703,9 +1703,9 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
         __syncthreads();
         if (found == -2) break;
         if (found == -1) {
-            // about 0.9 us between polls of an idle workgroup (measured
-            // 1.5 % faster than 0.2 us and than 3.4 us on the 64-replica
-            // workload; evaluation times were unchanged)
+            // about 0.9 us between polls of an idle workgroup (1.5 % faster
+            // than 0.2 us on short runs of the 64-replica workload, equal to
+            // 0.45-1.8 us on full runs; evaluation times do not change)
             __builtin_amdgcn_s_sleep(32);
             continue;
         }
