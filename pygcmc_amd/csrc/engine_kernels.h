@@ -1088,7 +1088,9 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
         L.misc[M_NG] = g;
         L.misc[M_N] = N;
         S.energy = et;
-        S.acc[mc.spi * 4 + kind] += na;
+        // counters only: an atomic add without return keeps the load of
+        // the old value off the serial path
+        if (na) count_add<false>(&S.acc[mc.spi * 4 + kind], (long long)na);
     }
 }
 
@@ -1179,7 +1181,9 @@ __device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int
         L.misc[M_NG] = g;
         L.misc[M_N] = N;
         S.energy = et;
-        S.acc[mc.spi * 4 + kind] += na;
+        // counters only: an atomic add without return keeps the load of
+        // the old value off the serial path
+        if (na) count_add<false>(&S.acc[mc.spi * 4 + kind], (long long)na);
     }
 }
 
@@ -1252,7 +1256,7 @@ __device__ inline void accept_move(const Rep &C, State &S, const Lds &L, const M
     if (!in_regs)
         for (int t = tid; t < k; t += NT) L.val[t] = (kind == K_TRN || kind == K_ROT) ? C.E[k + t] - C.E[t] : C.E[t];
     if (tid == 0) {
-        S.att[spi * 4 + kind] += 1;
+        count_add<false>(&S.att[spi * 4 + kind], 1ll);
         L.misc[M_ACC] = 0;
         L.misc[M_NG] = 0;
         L.misc[M_N] = mc.N;
@@ -1276,7 +1280,7 @@ __device__ inline void accept_move(const Rep &C, State &S, const Lds &L, const M
             }
             if (acc) {
                 S.energy = mc.energy - E;
-                S.acc[spi * 4 + kind] += 1;
+                count_add<false>(&S.acc[spi * 4 + kind], 1ll);
             }
         }
         __syncthreads();
@@ -1738,13 +1742,14 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             const unsigned long long ta0 = wall_clock64();
             const unsigned long long tpub = S.t_pub;
             const int step0 = S.step;
-            if (tid == 0) S.t_step += ta0 - tpub;
             const MoveCtx mc = move_ctx(C, S);
             accept_move<NT, false>(C, S, L, mc, true);
             const unsigned long long ta1 = wall_clock64();
             advance_and_publish<NT>(C, S, sch, L, true, mc.step, mc.spi);
             if (tid == 0) {
                 const unsigned long long ta2 = wall_clock64();
+                // timers after the next move is published, off its path
+                S.t_step += ta0 - tpub;
                 S.t_accept += ta1 - ta0;
                 S.t_prep += ta2 - ta1;
                 int row;
