@@ -229,3 +229,18 @@ def test_lockstep_bench_replica():
     gt, ot, g, r, st = _lockstep(s, 300, 'batched')
     err, ng, mism = _compare(gt, ot, g, r)
     print(f'bench replica 0 ({int(sizes[0])} fixed atoms): max |dE| {err:.3e}, {ng} groups, {mism} mismatches')
+
+
+def test_rejects_group_ids_beyond_cell_record_word():
+    # a mobile group's cell record carries its id in 24 bits: larger systems
+    # are refused at add_system, before any array is read
+    from pygcmc_amd import synth
+    s = synth.water_sphere(mcsteps=10, conf=3)
+    s['info'] = s['info'].copy()
+    s['info']['totalResNum'] = 1 << 24
+    eng = Engine()
+    try:
+        with pytest.raises(RuntimeError, match='totalResNum'):
+            eng.add_system(s)
+    finally:
+        eng.close()
