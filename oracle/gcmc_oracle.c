@@ -186,7 +186,29 @@ static void orc_apply(const float R[9], const float x[3], float y[3])
         y[j] = fmaf(x[2], R[3 * j + 2], fmaf(x[1], R[3 * j + 1], x[0] * R[3 * j + 0]));
 }
 
+/* Error codes of orc_run / orc_run_c (oracle.py raises on any of them). */
+#define ORC_E_STEPS (-1)  /* [first, first + nsteps) outside the move schedule */
+#define ORC_E_ALLOC (-2)  /* trial buffers could not be allocated */
+#define ORC_E_MOVE (-3)   /* a move code names a species outside fragTypeNum */
+#define ORC_E_K (-4)      /* a move's trial count exceeds ORC_MAXK or ORC_MAXF atoms */
+#define ORC_E_FOLLOW (-5) /* the followed trace is shorter or narrower than the run */
+#define ORC_E_STATE (-6)  /* a species count outside [0, maxNum] */
+#define ORC_E_ALLOC_CELLS (-7) /* the cell list could not be allocated */
+
 /* ---- system view -------------------------------------------------------- */
+/* Optional cell list over the centres of the active groups (a CPU timing
+   variant; the all-groups scan of calcEnergy is the default).  Cells are at
+   least rc + 0.05 A wide, so every group whose centre passes the FP32 test
+   lies in the 3 x 3 x 3 block of cells around the trial centre's cell (the
+   whole axis when it has fewer than 3 cells). */
+typedef struct {
+    int nc[3], ncell;
+    float h[3];
+    int *start;   /* [ncell + 1] */
+    int *group;   /* group ids sorted by cell */
+    int cap;      /* capacity of group */
+} orc_cells;
+
 typedef struct {
     const orc_info *info;
     orc_frag *frag;
@@ -195,6 +217,7 @@ typedef struct {
     const float *grid;
     const float *ff;
     const int *moves;
+    orc_cells *cells;   /* NULL: scan every group */
 } orc_sys;
 
 typedef struct {
@@ -270,6 +293,57 @@ static void orc_group_energy(const orc_sys *S, int g, const float c[3], int nf,
             *e += orc_pair(S, &S->atom[a], pose[f], tmpl[f].charge, tmpl[f].type);
 }
 
+static int orc_cell_of(const orc_cells *CL, const orc_info *info, const float p[3], int ci[3])
+{
+    for (int i = 0; i < 3; ++i) {
+        double w = (double)p[i] - (double)info->startxyz[i];
+        const double L = (double)info->cryst[i];
+        w -= floor(w / L) * L;
+        int c = (int)(w / (double)CL->h[i]);
+        ci[i] = c < 0 ? 0 : (c >= CL->nc[i] ? CL->nc[i] - 1 : c);
+    }
+    return (ci[0] * CL->nc[1] + ci[1]) * CL->nc[2] + ci[2];
+}
+
+/* bins the fixed groups and every species' active slots (a cell list of the
+   current state) */
+static int orc_cells_build(const orc_sys *S)
+{
+    orc_cells *CL = S->cells;
+    const orc_info *info = S->info;
+    const int nfix = S->frag[0].startRes;
+    int n = nfix;
+    for (int s = 0; s < info->fragTypeNum; ++s) n += S->frag[s].totalNum;
+    if (n > CL->cap) {
+        free(CL->group);
+        CL->group = (int *)malloc(sizeof(int) * (size_t)n);
+        if (!CL->group) { CL->cap = 0; return ORC_E_ALLOC_CELLS; }
+        CL->cap = n;
+    }
+    int *cellof = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *gid = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    if (!cellof || !gid) { free(cellof); free(gid); return ORC_E_ALLOC_CELLS; }
+    int m = 0;
+    for (int g = 0; g < nfix; ++g) gid[m++] = g;
+    for (int s = 0; s < info->fragTypeNum; ++s)
+        for (int j = 0; j < S->frag[s].totalNum; ++j) gid[m++] = S->frag[s].startRes + j;
+    memset(CL->start, 0, sizeof(int) * (size_t)(CL->ncell + 1));
+    for (int q = 0; q < m; ++q) {
+        int ci[3];
+        cellof[q] = orc_cell_of(CL, info, S->res[gid[q]].position, ci);
+        CL->start[cellof[q] + 1] += 1;
+    }
+    for (int c = 0; c < CL->ncell; ++c) CL->start[c + 1] += CL->start[c];
+    int *fill = (int *)malloc(sizeof(int) * (size_t)CL->ncell);
+    if (!fill) { free(cellof); free(gid); return ORC_E_ALLOC_CELLS; }
+    memcpy(fill, CL->start, sizeof(int) * (size_t)CL->ncell);
+    for (int q = 0; q < m; ++q) CL->group[fill[cellof[q]]++] = gid[q];
+    free(fill);
+    free(cellof);
+    free(gid);
+    return 0;
+}
+
 /* calcEnergy, gcmc_energy.h:142-204; exclude = moving residue or -1 */
 double orc_energy(const orc_sys *S, const float c[3], int species,
                   const float (*pose)[3], int exclude, int *ninc, uint32_t *hash)
@@ -278,6 +352,33 @@ double orc_energy(const orc_sys *S, const float c[3], int species,
     double e = 0.0;
     int n = 0;
     uint32_t h = 0;
+    if (S->cells) {
+        /* timing variant: the groups of the cells around c, same test */
+        const orc_cells *CL = S->cells;
+        int ci[3], lst[3][3], cnt[3];
+        orc_cell_of(CL, S->info, c, ci);
+        for (int i = 0; i < 3; ++i) {
+            if (CL->nc[i] < 3) {
+                cnt[i] = CL->nc[i];
+                for (int j = 0; j < CL->nc[i]; ++j) lst[i][j] = j;
+            } else {
+                cnt[i] = 3;
+                for (int j = 0; j < 3; ++j) lst[i][j] = (ci[i] + j - 1 + CL->nc[i]) % CL->nc[i];
+            }
+        }
+        for (int a = 0; a < cnt[0]; ++a)
+            for (int b = 0; b < cnt[1]; ++b)
+                for (int d = 0; d < cnt[2]; ++d) {
+                    const int cell = (lst[0][a] * CL->nc[1] + lst[1][b]) * CL->nc[2] + lst[2][d];
+                    for (int q = CL->start[cell]; q < CL->start[cell + 1]; ++q) {
+                        const int g = CL->group[q];
+                        if (g != exclude) orc_group_energy(S, g, c, F->num_atoms, pose, F->atoms, &e, &n, &h);
+                    }
+                }
+        if (ninc) *ninc = n;
+        if (hash) *hash = h;
+        return e;
+    }
     int nfix = S->frag[0].startRes;
     for (int g = 0; g < nfix; ++g)
         orc_group_energy(S, g, c, F->num_atoms, pose, F->atoms, &e, &n, &h);
@@ -425,6 +526,7 @@ typedef struct {
     double e_tot;
     long long attempts[64][4];
     long long accepts[64][4];
+    long long evals;      /* trial energy evaluations (translate/rotate count two) */
 } orc_stats;
 
 static inline int T_IDX(const orc_trace *T, int s, int k) { return s * T->max_k + k; }
@@ -436,6 +538,35 @@ static inline int T_IDX(const orc_trace *T, int s, int k) { return s * T->max_k 
  * states in lock-step after a rare disagreement.  Steps are indexed in the
  * trace arrays from 0 (= first).
  */
+
+
+/* Checks every argument the chain reads before any state changes: the steps
+   against the schedule length, each move code against the species count,
+   each species' count, atom count and trial count against the buffers, and a
+   followed trace against the steps and trials it must cover. */
+static int orc_check(const orc_sys *S, int nmoves, int first, int nsteps, const orc_trace *follow)
+{
+    const orc_info *info = S->info;
+    if (first < 0 || nsteps < 0 || nmoves < 0 || (long long)first + nsteps > (long long)nmoves) return ORC_E_STEPS;
+    if (info->fragTypeNum < 1 || info->fragTypeNum > 64) return ORC_E_MOVE;
+    for (int s = 0; s < info->fragTypeNum; ++s) {
+        const orc_frag *F = &S->frag[s];
+        if (F->totalNum < 0 || F->totalNum > F->maxNum) return ORC_E_STATE;
+        if (F->num_atoms < 1 || F->num_atoms > ORC_MAXF || F->confBias < 0 || F->confBias > ORC_MAXK)
+            return ORC_E_K;
+    }
+    for (int i = first; i < first + nsteps; ++i) {
+        const int mv = S->moves[i];
+        if (mv < 0 || mv / 4 >= info->fragTypeNum) return ORC_E_MOVE;
+    }
+    if (follow) {
+        if (follow->max_steps < nsteps) return ORC_E_FOLLOW;
+        for (int i = first; i < first + nsteps; ++i)
+            if (S->frag[S->moves[i] / 4].confBias > follow->max_k) return ORC_E_FOLLOW;
+    }
+    return 0;
+}
+
 int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
             const orc_trace *follow, orc_stats *st, int nthreads)
 {
@@ -453,13 +584,14 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
         unsigned char unused[ORC_MAXK];
     } orc_buf;
     orc_buf *bf = (orc_buf *)malloc(sizeof(orc_buf));
-    if (!bf) return -2;
+    if (!bf) return ORC_E_ALLOC;
     float (*centres)[3] = bf->centres;
     float (*poses)[ORC_MAXF][3] = bf->poses;
     double *e0 = bf->e0, *e1 = bf->e1, *w = bf->w;
     int *ninc0 = bf->ninc0, *ninc1 = bf->ninc1, *slots = bf->slots, *members = bf->members;
     uint32_t *ih0 = bf->ih0, *ih1 = bf->ih1;
     unsigned char *unused = bf->unused;
+    int cells_dirty = 1;
 
     for (int si = 0; si < nsteps; ++si) {
         int step = first + si;
@@ -474,9 +606,20 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
         else k = F->confBias < N ? F->confBias : N;
         if (k > ORC_MAXK) {
             free(bf);
-            return -1;
+            return ORC_E_K;
         }
-        if (st) st->attempts[sp][kind] += 1;
+        if (st) {
+            st->attempts[sp][kind] += 1;
+            st->evals += (kind >= 2) ? 2 * k : k;
+        }
+        if (S->cells && k > 0 && cells_dirty) {
+            const int rc = orc_cells_build(S);
+            if (rc) {
+                free(bf);
+                return rc;
+            }
+            cells_dirty = 0;
+        }
 
         /* picks for delete / translate / rotate */
         if (kind != 0 && k > 0) orc_picks(seed, step, N, k, slots);
@@ -553,6 +696,7 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
                 ngroups = 1;
                 if (facc) {
                     orc_delete_slot(S, sp, slots[0]);
+                    cells_dirty = 1;
                     nacc = 1;
                     if (st) { st->e_tot -= E; st->accepts[sp][kind] += 1; }
                 }
@@ -649,7 +793,7 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
                 for (int q = 0; q < n_ins; ++q) {
                     int t = ins_slots[q];
                     orc_write_pose(S, sp, slots[t], centres[t], (const float (*)[3])poses[t], 1);
-                    if (out && si < out->max_steps) out->idx[T_IDX(out, si, t)] = slots[t];
+                    if (out && si < out->max_steps && t < out->max_k) out->idx[T_IDX(out, si, t)] = slots[t];
                 }
             } else {
                 for (int t = 0; t < k; ++t)
@@ -659,6 +803,7 @@ int orc_run(orc_sys *S, uint32_t seed, int first, int nsteps, orc_trace *out,
                     }
             }
             if (st) st->accepts[sp][kind] += nacc;
+            if (nacc) cells_dirty = 1;
         }
         if (out && si < out->max_steps) {
             out->ngroups[si] = ngroups;
@@ -677,17 +822,58 @@ double orc_energy_pose(const orc_info *info, const orc_frag *frag, const orc_res
                        const float *centre, const float *pose, int exclude,
                        int *ninc, uint32_t *hash)
 {
-    orc_sys S = {info, (orc_frag *)frag, (orc_res *)res, (orc_atom *)atom, NULL, ff, NULL};
+    orc_sys S = {info, (orc_frag *)frag, (orc_res *)res, (orc_atom *)atom, NULL, ff, NULL, NULL};
     return orc_energy(&S, centre, species, (const float (*)[3])pose, exclude, ninc, hash);
 }
 
+/* nmoves = entries of moves[]; runs steps [first, first + nsteps) of it.
+   cells != 0 selects the cell-list scan (a timing variant; same inclusion
+   sets, energies summed in another order). */
 int orc_run_c(const orc_info *info, orc_frag *frag, orc_res *res, orc_atom *atom,
-              const float *grid, const float *ff, const int *moves, uint32_t seed,
+              const float *grid, const float *ff, const int *moves, int nmoves, uint32_t seed,
               int first, int nsteps, orc_trace *out, const orc_trace *follow,
-              orc_stats *st, int nthreads)
+              orc_stats *st, int nthreads, int cells)
 {
-    orc_sys S = {info, frag, res, atom, grid, ff, moves};
-    return orc_run(&S, seed, first, nsteps, out, follow, st, nthreads);
+    orc_sys S = {info, frag, res, atom, grid, ff, moves, NULL};
+    int rc = orc_check(&S, nmoves, first, nsteps, follow);
+    if (rc) return rc;
+    orc_cells CL;
+    memset(&CL, 0, sizeof(CL));
+    if (cells) {
+        CL.ncell = 1;
+        for (int i = 0; i < 3; ++i) {
+            const float L = info->cryst[i];
+            int n = (int)floor((double)L / ((double)info->cutoff + 0.05));
+            CL.nc[i] = n < 1 ? 1 : n;
+            CL.h[i] = L / (float)CL.nc[i];
+            CL.ncell *= CL.nc[i];
+        }
+        CL.start = (int *)malloc(sizeof(int) * (size_t)(CL.ncell + 1));
+        if (!CL.start) return ORC_E_ALLOC_CELLS;
+        S.cells = &CL;
+    }
+    rc = orc_run(&S, seed, first, nsteps, out, follow, st, nthreads);
+    free(CL.start);
+    free(CL.group);
+    return rc;
+}
+
+/* the trial pose of trial t of move `step` (kind 0 insert, 2 translate of
+   slot, 3 rotate of slot) and its test centre, for tests of the pose
+   generators */
+int orc_trial_pose(const orc_info *info, const orc_frag *frag, const orc_res *res, const orc_atom *atom,
+                   const float *grid, uint32_t seed, int step, int t, int species, int kind, int slot,
+                   float *centre, float *pose)
+{
+    orc_sys S = {info, (orc_frag *)frag, (orc_res *)res, (orc_atom *)atom, grid, NULL, NULL, NULL};
+    if (species < 0 || species >= info->fragTypeNum) return ORC_E_MOVE;
+    if (kind != 0 && (slot < 0 || slot >= frag[species].totalNum)) return ORC_E_STATE;
+    float (*p)[3] = (float (*)[3])pose;
+    if (kind == 0) orc_insert_pose(&S, seed, step, t, species, centre, p);
+    else if (kind == 2) orc_trn_pose(&S, seed, step, t, species, slot, centre, p);
+    else if (kind == 3) orc_rot_pose(&S, seed, step, t, species, slot, centre, p);
+    else return ORC_E_MOVE;
+    return 0;
 }
 
 uint32_t orc_group_hash_term(uint32_t g) { return orc_mix(g); }
@@ -765,7 +951,7 @@ double orc_adjoint(const orc_info *info, const orc_frag *frag, const orc_res *re
                    int exclude, const double *eps_kcal, const double *rmin_half,
                    const double *dq, double *gx, double *gq, double *ge, double *gr)
 {
-    orc_sys S = {info, (orc_frag *)frag, (orc_res *)res, (orc_atom *)atom, NULL, NULL, NULL};
+    orc_sys S = {info, (orc_frag *)frag, (orc_res *)res, (orc_atom *)atom, NULL, NULL, NULL, NULL};
     const int Y = info->ffYNum;
     double *sig = (double *)malloc(sizeof(double) * 3 * (size_t)Y);
     double *eps = sig + Y, *epk = sig + 2 * Y;

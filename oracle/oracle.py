@@ -44,8 +44,8 @@ def lib():
         L.orc_rotmat.argtypes = [P, ctypes.c_float, P]
         L.orc_energy_pose.argtypes = [P, P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P, P]
         L.orc_energy_pose.restype = ctypes.c_double
-        L.orc_run_c.argtypes = [P, P, P, P, P, P, P, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
-                                P, P, P, ctypes.c_int]
+        L.orc_run_c.argtypes = [P, P, P, P, P, P, P, ctypes.c_int, ctypes.c_uint32, ctypes.c_int,
+                                ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int]
         L.orc_run_c.restype = ctypes.c_int
         L.orc_picks.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
         L.orc_species_B.argtypes = [P, P]
@@ -56,6 +56,9 @@ def lib():
         L.orc_pose_hash.restype = ctypes.c_uint32
         L.orc_adjoint.argtypes = [P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P, P, P, P, P, P, P]
         L.orc_adjoint.restype = ctypes.c_double
+        L.orc_trial_pose.argtypes = [P, P, P, P, P, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, P, P]
+        L.orc_trial_pose.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -131,6 +134,20 @@ def adjoint(sysd, species, centre, pose, eps, rmin, dq=None, exclude=-1):
     return e, gx, gq, ge, gr
 
 
+def trial_pose(sysd, step, t, species, kind, slot=-1):
+    """(centre[3], pose[nf, 3]) of trial t of move `step` for the current state:
+    kind 0 inserts (randomFragment), 2 translates and 3 rotates slot `slot`."""
+    nf = int(sysd['frag'][species]['num_atoms'])
+    c = np.zeros(3, np.float32)
+    p = np.zeros((nf, 3), np.float32)
+    rc = lib().orc_trial_pose(_p(sysd['info']), _p(sysd['frag']), _p(sysd['res']), _p(sysd['atoms']),
+                              _p(sysd['grid']), int(sysd['info'][0]['seed']), step, t, species, kind, slot,
+                              _p(c), _p(p))
+    if rc != 0:
+        raise ValueError(f'trial_pose refused ({rc})')
+    return c, p
+
+
 def energy_pose(sysd, species, centre, pose, exclude=-1):
     """FP64 energy of a fragment pose around `centre` against the current state."""
     c = np.ascontiguousarray(centre, np.float32)
@@ -151,7 +168,7 @@ class Trace(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [('e_tot', ctypes.c_double), ('attempts', (ctypes.c_longlong * 4) * 64),
-                ('accepts', (ctypes.c_longlong * 4) * 64)]
+                ('accepts', (ctypes.c_longlong * 4) * 64), ('evals', ctypes.c_longlong)]
 
 
 TRACE_FIELDS = {
@@ -186,18 +203,44 @@ def copy_system(sysd):
     return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sysd.items()}
 
 
-def run(sysd, first=0, nsteps=None, trace=None, follow=None, nthreads=1, stats=None):
-    """Run the FP64 chain in place on sysd's arrays.  Returns (trace, stats)."""
-    nsteps = int(sysd['info'][0]['mcsteps']) - first if nsteps is None else nsteps
+RUN_ERRORS = {
+    -1: 'steps [first, first + nsteps) outside the move schedule',
+    -2: 'trial buffers could not be allocated',
+    -3: 'a move code names a species outside fragTypeNum',
+    -4: 'a trial count or fragment atom count exceeds the oracle buffers',
+    -5: 'the followed trace is shorter or narrower than the run',
+    -6: 'a species count outside [0, maxNum]',
+    -7: 'the cell list could not be allocated',
+}
+
+
+def run(sysd, first=0, nsteps=None, trace=None, follow=None, nthreads=1, stats=None, cells=False):
+    """Run the FP64 chain in place on sysd's arrays.  Returns (trace, stats).
+
+    cells=True scans a cell list of group centres instead of every group (a
+    CPU timing variant with the same inclusion sets; energies are summed in
+    another order).  Raises ValueError, before changing any state, when the
+    steps leave the move schedule, a move code or species count is invalid,
+    or a trace is too small for the run."""
+    moves = sysd['moves']
+    if not (isinstance(moves, np.ndarray) and moves.dtype == np.int32 and moves.flags['C_CONTIGUOUS']):
+        raise TypeError('moves must be a C-contiguous int32 array')
+    nsteps = int(sysd['info'][0]['mcsteps']) - first if nsteps is None else int(nsteps)
+    first = int(first)
+    for name, tr in (('trace', trace), ('follow', follow)):
+        if tr is not None and tr['k'].shape[0] < nsteps:
+            raise ValueError(f'{name} holds {tr["k"].shape[0]} steps, the run needs {nsteps}')
     st = stats if stats is not None else Stats()
     ts = _trace_struct(trace) if trace is not None else None
     fs = _trace_struct(follow) if follow is not None else None
     rc = lib().orc_run_c(_p(sysd['info']), _p(sysd['frag']), _p(sysd['res']), _p(sysd['atoms']),
-                         _p(sysd['grid']), _p(sysd['ff']), _p(sysd['moves']),
+                         _p(sysd['grid']), _p(sysd['ff']), _p(moves), len(moves),
                          int(sysd['info'][0]['seed']), first, nsteps,
                          ctypes.byref(ts) if ts is not None else None,
                          ctypes.byref(fs) if fs is not None else None,
-                         ctypes.byref(st), nthreads)
+                         ctypes.byref(st), nthreads, 1 if cells else 0)
     if rc != 0:
-        raise RuntimeError(f'oracle run failed ({rc})')
+        msg = RUN_ERRORS.get(rc, 'unknown error')
+        raise ValueError(f'oracle run refused ({rc}): {msg} (first={first}, nsteps={nsteps}, '
+                         f'schedule={len(moves)} moves)')
     return trace, st
