@@ -1,21 +1,33 @@
 """Benchmark: GCMC MC steps/s and trial energy evaluations/s on one GPU.
 
-Workload (default): 64 independent GCMC systems whose fixed-atom counts are
+Workload: 64 independent GCMC systems whose fixed-atom counts are
 log-uniform in 5,000-60,000 (box 40*(N/2000)^(1/3) A, fixed-atom density of the
 2,000-atom sphere system), each with its own seed and muex and the 8 rigid
 fragment types + TIP3P water (confBias 400/15/10/20/15/300/100/200/2), 10,000
-moves each, FP32 pair energies with compensated sums and FP64 totals and
-acceptance, all systems in one persistent kernel.  Synthetic data (seeded).
+moves each, FP32 pair energies with compensated sums (Coulomb terms from the
+exactly formed FP32 displacement in FP64), FP64 totals and acceptance, all
+systems in one persistent kernel.  Synthetic data (seeded).
+
+One bench step is one whole pass of that workload: every system returned to
+its initial state on the device (device-to-device copies from a snapshot
+taken after upload), then all 10,000 moves of all 64 systems (640,000 MC
+steps).  --warmup passes are untimed, --steps passes are timed; every pass
+computes the same chains, and the bench checks that they end in the same
+state.
 
 Prints one JSON line.  `value` is the batched form's aggregate MC steps/s over
-all systems with inputs resident in HBM; the plain form (one move of one
-system per launch pair) is timed on the same moves beside it.
+all systems with inputs resident in HBM.  Beside it: the plain form (one move
+of one system per launch pair) and the batched form on the same first moves,
+the graph-launched form, the roofline of the persistent kernel, a lock-step
+check of the first moves of every system against the FP64 oracle (the CPU
+baseline leg, timed on the host cores), the adjoint and set-up from files.
 """
 
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -23,22 +35,24 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak (packed FP32)
+FP32_UNPACKED_TFLOPS = 78.65  # the same without packed FP32 (this build emits none)
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_PAIR = 32         # algorithmic count per fragment-environment pair
 BYTES_PER_ITEM = 20        # per included env atom / included group / fragment atom
+KCAL_KJ = 4.184e-4         # 1e-4 kcal/mol in kJ/mol
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_replicas(n, steps):
+def build_replicas(n, moves):
     from pygcmc_amd import synth
     sizes = synth.replica_sizes(n)
     out = []
     for r, na in enumerate(sizes):
-        out.append(synth.replica_system(r, int(na), mcsteps=steps))
+        out.append(synth.replica_system(r, int(na), mcsteps=moves))
     return out, sizes
 
 
@@ -63,55 +77,159 @@ def counters_sum(eng, R):
     return tot
 
 
-def cpu_baseline(states, first, budget_s, nthreads=1):
-    """The FP64 oracle (reference algorithm: all groups scanned per trial;
-    nthreads OpenMP threads over the trials of a move) continuing the same
-    chains from step `first`."""
-    from oracle import oracle as O
-    done = 0
-    t0 = time.perf_counter()
-    for s in states:
-        step = first
-        while time.perf_counter() - t0 < budget_s and step < len(s['moves']):
-            O.run(s, first=step, nsteps=1, nthreads=nthreads)
-            step += 1
-            done += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return done / dt, done, dt
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
 
 
-def parity_sample(system, nsteps):
-    """Max |dE| of the GPU against the FP64 oracle on one system (lock-step)."""
-    from oracle import oracle as O
+def host_threads():
+    """Threads for the CPU legs: the host's cores, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def e_tol(e):
+    e = np.abs(e)
+    return np.where(e <= 200.0, KCAL_KJ, np.maximum(KCAL_KJ, 1e-5 * e))
+
+
+def gpu_traces(systems, moves, tuning):
+    """The first `moves` moves of every system in one batched launch, traced."""
     from pygcmc_amd.engine import Engine
-    g = copies([system])[0]
-    k = int(g['frag']['confBias'].max())
-    eng = Engine()
-    eng.add_system(g)
+    eng = Engine(**tuning)
+    gsys = copies(systems)
+    ks = []
+    for s in gsys:
+        eng.add_system(s)
+        ks.append(int(s['frag']['confBias'].max()))
     eng.upload()
-    eng.set_trace(0, nsteps, k)
-    eng.run(nsteps, 'batched')
-    tr = eng.get_trace(0)
+    for r, k in enumerate(ks):
+        eng.set_trace(r, moves, k)
+    eng.run(moves, 'batched')
+    traces = [eng.get_trace(r) for r in range(len(gsys))]
+    for r in range(len(gsys)):
+        eng.download(r)
     eng.close()
-    ref = copies([system])[0]
-    ot = O.new_trace(nsteps, k)
-    O.run(ref, first=0, nsteps=nsteps, trace=ot, follow=tr)
-    m = np.arange(k)[None, :] < tr['k'][:, None]
-    e_ref = ot['e'][m]
-    err = np.abs(tr['e'][m] - e_ref)
-    band = np.abs(e_ref) <= 200.0          # the band the 1e-4 kcal/mol bar applies to
-    gm = np.arange(k)[None, :] < tr['ngroups'][:, None]
-    agree = float((tr['gacc'][gm] == ot['gacc'][gm]).mean()) if gm.any() else 1.0
-    rel_hi = float((err[~band] / np.abs(e_ref[~band])).max()) if (~band).any() else 0.0
-    return float(err[band].max()) if band.any() else 0.0, agree, int(gm.sum()), rel_hi
+    return traces, gsys
 
 
-def measured_traffic(steps, replicas, tuning):
-    """HBM bytes of the timed k_batched launch from the committed rocprofv3
-    --pmc record (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known
-    size), when that record was measured on this same workload; else None."""
+def cpu_lockstep(systems, traces, gsys, moves, nthreads):
+    """CPU baseline and parity sample in one: the FP64 oracle (the
+    reference's algorithm, every group tested per trial) runs the first
+    `moves` moves of every system on `nthreads` host threads (systems spread
+    over the threads), taking the GPU's picks so the chains stay in step, and
+    its trial energies, poses, inclusion sets, decisions and final states are
+    compared with the GPU's."""
+    from oracle import oracle as O
+    R = len(systems)
+    refs = copies(systems)
+    otr = [None] * R
+    res = [None] * R
+    lock = threading.Lock()
+    nxt = [0]
+
+    def worker():
+        while True:
+            with lock:
+                r = nxt[0]
+                nxt[0] += 1
+            if r >= R:
+                return
+            k = traces[r]['centre'].shape[1]
+            ot = O.new_trace(moves, k)
+            O.run(refs[r], first=0, nsteps=moves, trace=ot, follow=traces[r])
+            otr[r] = ot
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker) for _ in range(nthreads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    worst = 0.0
+    rel_hi = 0.0
+    groups = 0
+    mism = 0
+    exact = True
+    over = 0
+    for r in range(R):
+        gt, ot = traces[r], otr[r]
+        km = np.arange(gt['centre'].shape[1])[None, :] < gt['k'][:, None]
+        exact &= bool(np.array_equal(gt['k'], ot['k']) and
+                      np.array_equal(gt['centre'][km], ot['centre'][km]) and
+                      np.array_equal(gt['posehash'][km], ot['posehash'][km]) and
+                      np.array_equal(gt['ninc'][km], ot['ninc'][km]) and
+                      np.array_equal(gt['inchash'][km], ot['inchash'][km]) and
+                      np.array_equal(gsys[r]['atoms'].view(np.uint8), refs[r]['atoms'].view(np.uint8)) and
+                      np.array_equal(gsys[r]['frag']['totalNum'], refs[r]['frag']['totalNum']))
+        eo = ot['e'][km]
+        err = np.abs(gt['e'][km] - eo)
+        band = np.abs(eo) <= 200.0
+        if band.any():
+            worst = max(worst, float(err[band].max()))
+        if (~band).any():
+            rel_hi = max(rel_hi, float((err[~band] / np.abs(eo[~band])).max()))
+        over += int((err > e_tol(eo)).sum())
+        gm = np.arange(gt['gpick'].shape[1])[None, :] < gt['ngroups'][:, None]
+        groups += int(gm.sum())
+        mism += int(((gt['gpick'][gm] != ot['gpick'][gm]) | (gt['gacc'][gm] != ot['gacc'][gm])).sum())
+    return {'seconds': dt, 'moves': R * moves, 'max_abs_dE_kJ': worst, 'rel_err_above_200': rel_hi,
+            'energies_over_bar': over, 'groups': groups, 'group_mismatches': mism,
+            'bit_exact_poses_sets_states': exact}
+
+
+def cpu_round_robin(systems, budget_s, nthreads, cells, chunk=5):
+    """The oracle running the systems' chains from their first move, in
+    chunks of `chunk` moves taken round-robin over the systems, on `nthreads`
+    threads, until `budget_s` has passed: MC steps done per second.
+    cells=True scans a cell list of group centres instead of every group."""
+    from oracle import oracle as O
+    R = len(systems)
+    states = copies(systems)
+    pos = [0] * R
+    done = [0] * nthreads
+    t_end = time.perf_counter() + budget_s
+
+    def worker(tid):
+        while time.perf_counter() < t_end:
+            moved = False
+            for r in range(tid, R, nthreads):
+                if time.perf_counter() >= t_end:
+                    break
+                n = min(chunk, len(states[r]['moves']) - pos[r])
+                if n <= 0:
+                    continue
+                O.run(states[r], first=pos[r], nsteps=n, cells=cells)
+                pos[r] += n
+                done[tid] += n
+                moved = True
+            if not moved:
+                return
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=worker, args=(i,)) for i in range(nthreads)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    dt = time.perf_counter() - t0
+    return sum(done) / dt, sum(done), dt, max(pos)
+
+
+def measured_traffic(moves, replicas, tuning):
+    """HBM bytes of one k_batched launch (one pass of the workload) from the
+    committed rocprofv3 --pmc record (FETCH_SIZE and WRITE_SIZE scaled by a
+    copy kernel of known size), when that record was measured on this same
+    workload and setting; else None."""
     path = os.path.join(ROOT, 'profiles', 'k_batched_64replicas.pmc.txt')
     if not os.path.exists(path):
         return None, None
@@ -120,10 +238,11 @@ def measured_traffic(steps, replicas, tuning):
         if '=' in line and not line.startswith('#'):
             k, v = line.strip().split('=', 1)
             rec[k] = v
-    same = (int(rec.get('steps', -1)) == steps and int(rec.get('replicas', -1)) == replicas and
+    same = (int(rec.get('moves', -1)) == moves and int(rec.get('replicas', -1)) == replicas and
             float(rec.get('cell_size', -1)) == float(tuning.get('cell_size', 0)) and
             int(rec.get('wg_size', -1)) == int(tuning.get('wg_size', 0) or 256) and
-            int(rec.get('pair_lds', 1)) == int(tuning.get('pair_lds', 1)))
+            int(rec.get('pair_lds', 1)) == int(tuning.get('pair_lds', 1)) and
+            int(rec.get('fp32_coulomb', 0)) == int(tuning.get('fp32_coulomb', 0)))
     if not same:
         return None, None
     return float(rec['traffic_bytes']), 'profiles/k_batched_64replicas.pmc.txt (rocprofv3 --pmc, same workload)'
@@ -223,13 +342,16 @@ def adjoint_bench(npose, reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=9800)
-    ap.add_argument('--warmup', type=int, default=200)
+    ap.add_argument('--steps', type=int, default=3, help='timed passes of the whole workload')
+    ap.add_argument('--warmup', type=int, default=1, help='untimed passes before them')
     ap.add_argument('--replicas', type=int, default=64)
-    ap.add_argument('--plain-steps', type=int, default=100)
-    ap.add_argument('--cpu-seconds', type=float, default=15.0)
-    ap.add_argument('--parity-steps', type=int, default=300)
-    ap.add_argument('--mode', default='graph', choices=['batched', 'graph'])
+    ap.add_argument('--moves', type=int, default=10000, help='moves per system in one pass')
+    ap.add_argument('--plain-moves', type=int, default=200)
+    ap.add_argument('--graph-passes', type=int, default=2)
+    ap.add_argument('--lockstep-moves', type=int, default=200,
+                    help='first moves of every system run by the FP64 oracle (CPU baseline + parity)')
+    ap.add_argument('--cpu-seconds', type=float, default=10.0,
+                    help='budget of each round-robin CPU leg (one core; cell-list variant on all cores)')
     # defaults: the setting pygcmc_amd.autotune picks on this workload (its
     # timings differ by under 2% across cell sizes 5-7.5 A); --autotune 1
     # searches again
@@ -237,9 +359,10 @@ def main():
     ap.add_argument('--wgs', type=int, default=0)
     ap.add_argument('--wg-size', type=int, default=256)
     ap.add_argument('--pair-lds', type=int, default=0)
+    ap.add_argument('--fp32-coulomb', type=int, default=0)
     ap.add_argument('--kernel-only', action='store_true',
-                    help='only the timed batched run (for profiler runs): no graph, plain, parity, '
-                         'adjoint or CPU legs')
+                    help='only the timed passes (for profiler runs): no graph, plain, lock-step, '
+                         'adjoint, set-up or CPU legs')
     ap.add_argument('--adjoint-poses', type=int, default=4096)
     ap.add_argument('--setup-bench', type=int, default=1,
                     help='time set-up of a 4wp7-sized system from PDB/TOP/force-field files')
@@ -259,90 +382,119 @@ def main():
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported in this build '
               '(independent systems would shard one group per GPU, see DESIGN.md)', file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0 or args.moves < 1:
+        print('--steps >= 1, --warmup >= 0 and --moves >= 1 are required', file=sys.stderr)
+        sys.exit(2)
 
-    total = args.warmup + args.steps
+    M = args.moves
     t0 = time.time()
-    systems, sizes = build_replicas(args.replicas, total)
+    systems, sizes = build_replicas(args.replicas, M)
     R = len(systems)
-    log(f'built {R} systems ({int(sizes.sum())} fixed atoms) in {time.time() - t0:.1f}s')
-    tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs, wg_size=args.wg_size, pair_lds=args.pair_lds)
+    log(f'built {R} systems ({int(sizes.sum())} fixed atoms, {M} moves each) in {time.time() - t0:.1f}s')
+    tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs, wg_size=args.wg_size, pair_lds=args.pair_lds,
+                  fp32_coulomb=args.fp32_coulomb)
     tune_table = None
     if args.autotune:
         from pygcmc_amd.autotune import autotune
         t1 = time.time()
         tuning, tune_table = autotune(systems, log=log)
+        tuning['fp32_coulomb'] = args.fp32_coulomb
         log(f'autotune: {tuning} in {time.time() - t1:.1f}s')
 
-    # batched form over the whole schedule
     eng = make_engine(copies(systems), tuning)
-    if args.warmup:
-        eng.run(args.warmup, 'batched')
-    states_w = [eng.download(r, copies([systems[r]])[0]) for r in range(R)]
-    c0 = counters_sum(eng, R)
-    ms_total = eng.run(args.steps, 'batched')
-    ms_kernel = eng.last_kernel_ms()
-    c1 = counters_sum(eng, R)
-    launch = eng.launch_info()
-    eng.close()
-    steps_done = R * args.steps
-    value = steps_done / (ms_total / 1e3)
-    evals, pairs, atoms, groups, fatoms = c1 - c0
-    log(f'batched: {steps_done} steps in {ms_total:.1f} ms (kernel {ms_kernel:.1f} ms)')
+    eng.snapshot()
 
-    tflops = FLOP_PER_PAIR * pairs / (ms_kernel / 1e3) / 1e12
-    gbs = BYTES_PER_ITEM * (atoms + groups + fatoms) / (ms_kernel / 1e3) / 1e9
+    def one_pass(mode='batched'):
+        eng.restore()
+        ms = eng.run(M, mode)
+        for r in range(R):
+            if eng.step(r) != M:
+                raise RuntimeError(f'system {r} ran {eng.step(r)} of {M} moves')
+        return ms
+
+    energies_w = None
+    for w in range(args.warmup):
+        ms = one_pass()
+        log(f'warm-up pass {w + 1}/{args.warmup}: {ms:.1f} ms')
+        energies_w = [eng.stats(r)['energy'] for r in range(R)]
+    run_ms, kern_ms = [], []
+    eng.synchronize()
+    t_wall = time.perf_counter()
+    for k in range(args.steps):
+        run_ms.append(one_pass())
+        kern_ms.append(eng.last_kernel_ms())
+    eng.synchronize()
+    wall_s = time.perf_counter() - t_wall
+    # every pass computes the same chains from the same start
+    energies0 = [eng.stats(r)['energy'] for r in range(R)]
+    if energies_w is not None and energies_w != energies0:
+        raise RuntimeError('a timed pass ended in another state than the warm-up pass')
+    cnt = counters_sum(eng, R)
+    launch = eng.launch_info()
+    steps_per_pass = R * M
+    value = steps_per_pass * args.steps / wall_s
+    evals, pairs, atoms, groups, fatoms = cnt
+    kernel_ms = float(np.mean(kern_ms))
+    log(f'timed: {args.steps} passes x {steps_per_pass} MC steps in {wall_s:.3f} s '
+        f'(HIP events: run {np.mean(run_ms):.1f} ms, kernel {kernel_ms:.1f} ms per pass)')
+
+    tflops = FLOP_PER_PAIR * pairs / (kernel_ms / 1e3) / 1e12
+    alg_bytes = BYTES_PER_ITEM * (atoms + groups + fatoms)
+    gbs = alg_bytes / (kernel_ms / 1e3) / 1e9
+    traffic, traffic_src = measured_traffic(M, R, tuning)
     if args.kernel_only:
-        print(json.dumps({'value': round(value, 1), 'unit': 'MC steps/s', 'kernel_ms': ms_kernel,
-                          'tflops': tflops, 'algorithmic_GBps': gbs, 'launch': launch}), flush=True)
+        eng.close()
+        print(json.dumps({'value': round(value, 1), 'unit': 'MC steps/s', 'kernel_ms': kern_ms,
+                          'run_ms': run_ms, 'tflops': tflops, 'algorithmic_GBps': gbs,
+                          'algorithmic_bytes': alg_bytes, 'launch': launch}), flush=True)
         return
 
-    # graph replay of the same chains (timing only, from the warm state)
+    # the same passes launched from a captured hipGraph (first launch captures)
     graph_rate = None
-    if args.mode == 'graph':
-        eg = make_engine(copies(states_w), tuning)
-        gsteps = min(args.steps, 2000)
-        ms_g = 0.0
-        done = 0
-        while done < gsteps:
-            n = min(500, gsteps - done)
-            ms_g += eg.run(n, 'graph')
-            done += n
-        eg.close()
-        graph_rate = R * gsteps / (ms_g / 1e3)
-        log(f'graph: {R * gsteps} steps in {ms_g:.1f} ms')
+    if args.graph_passes > 0:
+        one_pass('graph')
+        gms = [one_pass('graph') for _ in range(args.graph_passes)]
+        graph_rate = steps_per_pass / (np.mean(gms) / 1e3)
+        same = [eng.stats(r)['energy'] for r in range(R)] == energies0
+        log(f'graph: {args.graph_passes} passes, {np.mean(gms):.1f} ms each; same chains: {same}')
+        if not same:
+            raise RuntimeError('graph passes ended in another state than the batched passes')
 
     # plain vs batched on the same first moves
-    ps = args.plain_steps
-    ep = make_engine(copies(systems), tuning)
-    ms_plain = ep.run(ps, 'plain')
-    ep.close()
-    eb = make_engine(copies(systems), tuning)
-    ms_b = eb.run(ps, 'batched')
-    eb.close()
-    plain_rate = R * ps / (ms_plain / 1e3)
-    batched_same = R * ps / (ms_b / 1e3)
-    log(f'plain: {R * ps} steps in {ms_plain:.1f} ms; batched same moves {ms_b:.1f} ms')
+    P = min(args.plain_moves, M)
+    eng.restore()
+    ms_plain = eng.run(P, 'plain')
+    plain_e = [eng.stats(r)['energy'] for r in range(R)]
+    eng.restore()
+    ms_b = eng.run(P, 'batched')
+    same_pb = [eng.stats(r)['energy'] for r in range(R)] == plain_e
+    eng.close()
+    plain_rate = R * P / (ms_plain / 1e3)
+    batched_same = R * P / (ms_b / 1e3)
+    log(f'plain: {R * P} steps in {ms_plain:.1f} ms; batched same moves {ms_b:.1f} ms; same chains: {same_pb}')
 
-    # accuracy on one system against the FP64 oracle
-    max_err, agree, ngroups, rel_hi = parity_sample(systems[0], args.parity_steps)
-    log(f'parity: max |dE| {max_err:.2e} kJ/mol (|E| <= 200), rel {rel_hi:.1e} above; decisions agree '
-        f'{agree:.6f} over {ngroups} groups')
+    # CPU baseline + parity: the FP64 oracle on the first moves of every system
+    nth = host_threads()
+    Lm = min(args.lockstep_moves, M)
+    t1 = time.time()
+    traces, gsys = gpu_traces(systems, Lm, tuning)
+    lk = cpu_lockstep(systems, traces, gsys, Lm, nth)
+    del traces, gsys
+    log(f'lock-step: {lk} (with GPU traces {time.time() - t1:.1f} s)')
+    c1_rate, c1_done, c1_dt, c1_pos = cpu_round_robin(systems, args.cpu_seconds, 1, False)
+    log(f'cpu 1 thread, all groups: {c1_done} steps in {c1_dt:.1f} s')
+    cc_rate, cc_done, cc_dt, cc_pos = cpu_round_robin(systems, args.cpu_seconds, nth, True)
+    log(f'cpu {nth} threads, cell list: {cc_done} steps in {cc_dt:.1f} s')
 
-    # CPU baseline: the oracle continuing the warm chains
-    cpu_rate, cpu_done, cpu_dt = cpu_baseline(copies(states_w[:4]), args.warmup, args.cpu_seconds)
-    log(f'cpu: {cpu_done} steps in {cpu_dt:.1f}s')
-    ncpu = min(os.cpu_count() or 1, 16)
-    cpu_rate_n, cpu_done_n, cpu_dt_n = cpu_baseline(copies(states_w[:4]), args.warmup, args.cpu_seconds / 3, ncpu)
-    log(f'cpu ({ncpu} threads): {cpu_done_n} steps in {cpu_dt_n:.1f}s')
-
-    traffic, traffic_src = measured_traffic(args.steps, R, tuning)
     adj = adjoint_bench(args.adjoint_poses) if args.adjoint_poses > 0 else None
+    if adj:
+        log(f"adjoint: {adj['poses']} poses fwd+bwd in {adj['ms']:.2f} ms")
     setup = setup_bench() if args.setup_bench else None
     if setup:
         log(f"set-up from files: {setup}")
-    if adj:
-        log(f"adjoint: {adj['poses']} poses fwd+bwd in {adj['ms']:.2f} ms")
 
+    cpu_rate = lk['moves'] / lk['seconds']
+    model = cpu_model()
     out = {
         'metric': 'MC steps/sec + trial dE evals/sec, batched vs plain, at |dE| err <= 1e-4 kcal/mol vs FP64',
         'value': round(value, 1),
@@ -350,47 +502,71 @@ def main():
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': ms_total / args.steps,
+        'ms_per_step': wall_s * 1e3 / args.steps,
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'fp32 pairs + fp64 totals',
+        'dtype': 'fp32 (LJ pair terms; Coulomb pair terms fp64 from the exact fp32 displacement; fp64 sums '
+                 'and acceptance)' if not args.fp32_coulomb else 'fp32 pair terms, compensated sums; fp64 '
+                                                                 'sums and acceptance',
         'data': 'synthetic (seeded)',
-        'config': {'workload': f'{R} uneven GCMC replicas (5k-60k fixed atoms, 8 fragments + water, '
-                               f'{total} moves each)', 'replicas': R,
-                   'fixed_atoms_total': int(sizes.sum()), 'moves_per_replica': total,
-                   'parallelism': 'replicas in one persistent kernel'},
-        'trial_evals_per_s': round(evals / (ms_total / 1e3), 1),
-        'pairs_per_s': pairs / (ms_total / 1e3),
+        'config': {'workload': f'{R} uneven GCMC replicas (5k-60k fixed atoms, 8 fragments + water), '
+                               f'{M} moves each per step; one step = one pass from the initial state',
+                   'replicas': R, 'fixed_atoms_total': int(sizes.sum()), 'moves_per_replica': M,
+                   'mc_steps_per_step': steps_per_pass, 'parallelism': 'replicas in one persistent kernel'},
+        'trial_evals_per_s': round(evals * args.steps / wall_s, 1),
+        'pairs_per_s': pairs * args.steps / wall_s,
+        'pass_ms_hip_events': [round(x, 2) for x in run_ms],
+        'kernel_ms_hip_events': [round(x, 2) for x in kern_ms],
         'graph_steps_per_s': None if graph_rate is None else round(graph_rate, 1),
         'plain_steps_per_s': round(plain_rate, 1),
         'batched_steps_per_s_same_moves': round(batched_same, 1),
         'batched_over_plain': round(batched_same / plain_rate, 2),
-        'max_abs_dE_kJ': max_err,
-        'max_abs_dE_kcal': max_err / 4.184,
+        'plain_sample': f'first {P} moves of every system; plain and batched chains identical: {same_pb}',
+        'max_abs_dE_kJ': lk['max_abs_dE_kJ'],
+        'max_abs_dE_kcal': lk['max_abs_dE_kJ'] / 4.184,
         'dE_band': '|E| <= 200 kJ/mol (Boltzmann weight > e^-80); above it max relative error '
-                   f'{rel_hi:.1e}',
-        'parity_sample': f'system 0, first {args.parity_steps} moves, lock-step with the FP64 oracle',
-        'decision_agreement': agree,
+                   f"{lk['rel_err_above_200']:.1e}",
+        'parity_sample': f'first {Lm} moves of all {R} systems in one batched launch, lock-step with the FP64 '
+                         f"oracle: {lk['groups']} selection groups, {lk['group_mismatches']} mismatches, "
+                         f"{lk['energies_over_bar']} energies over the bar, poses / inclusion sets / final "
+                         f"states bit-exact: {lk['bit_exact_poses_sets_states']}",
+        'decision_agreement': 1.0 - lk['group_mismatches'] / max(1, lk['groups']),
         'launch': launch,
         'tuning': tuning,
         'autotune_ms': None if tune_table is None else [[t, round(ms, 3)] for t, ms in tune_table],
         'adjoint': adj,
         'setup_from_files': setup,
-        'roofline': {'bound': 'mfma', 'pipe': 'fp32-valu', 'kernel': 'k_batched',
-                     'achieved': round(tflops, 3), 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': round(tflops / FP32_PEAK_TFLOPS, 4), 'traffic': traffic,
+        'roofline': {'bound': 'hbm', 'kernel': 'k_batched',
+                     'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(gbs / HBM_PEAK_GBS, 4), 'traffic': traffic,
                      'traffic_source': traffic_src,
-                     'hbm_GBps': None if traffic is None else round(traffic / (ms_kernel / 1e3) / 1e9, 1),
-                     'hbm_frac': None if traffic is None else round(traffic / (ms_kernel / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
-                     'algorithmic_GBps': round(gbs, 1), 'kernel_ms': round(ms_kernel, 2),
+                     'algorithmic_bytes_per_launch': alg_bytes,
+                     'bytes_rule': '20 B per included environment atom, included group and fragment atom, '
+                                   'per evaluation',
+                     'hbm_GBps_measured': None if traffic is None else round(traffic / (kernel_ms / 1e3) / 1e9, 1),
+                     'kernel_ms': round(kernel_ms, 2),
+                     'limiter': 'latency of L1 misses served by L2 in each CU vector-memory pipeline '
+                                '(rocprofv3 counters), not HBM bandwidth or ALU throughput',
+                     'fp32': {'achieved_TFLOPs': round(tflops, 3), 'peak_TFLOPs': FP32_PEAK_TFLOPS,
+                              'frac': round(tflops / FP32_PEAK_TFLOPS, 4),
+                              'ceiling_without_packed_fp32_TFLOPs': FP32_UNPACKED_TFLOPS,
+                              'flop_rule': '32 FLOP per fragment-environment atom pair'},
                      'pairs': int(pairs), 'evals': int(evals)},
-        'cpu_baseline': {'value': round(cpu_rate, 2), 'unit': 'MC steps/s', 'cores': 1, 'kind': 'port',
-                         'sample': f'FP64 oracle, {cpu_done} moves of systems 0-3 continuing from '
-                                   f'move {args.warmup} ({cpu_dt:.1f} s)'},
-        'cpu_baseline_threads': {'value': round(cpu_rate_n, 2), 'unit': 'MC steps/s', 'cores': ncpu,
-                                 'kind': 'port', 'sample': f'same, OpenMP over the trials of a move, '
-                                                           f'{cpu_done_n} moves ({cpu_dt_n:.1f} s)'},
+        'cpu_baseline': {'value': round(cpu_rate, 2), 'unit': 'MC steps/s', 'cores': nth, 'kind': 'port',
+                         'cpu': model,
+                         'sample': f"FP64 oracle (reference algorithm: every group tested per trial), first {Lm} "
+                                   f"moves of all {R} systems = {lk['moves']} MC steps, systems spread over {nth} "
+                                   f"threads, {lk['seconds']:.1f} s (the lock-step parity run)"},
+        'cpu_baseline_1core': {'value': round(c1_rate, 2), 'unit': 'MC steps/s', 'cores': 1, 'kind': 'port',
+                               'cpu': model,
+                               'sample': f'same oracle, one thread, chunks of 5 moves round-robin over the '
+                                         f'systems from move 0: {c1_done} MC steps in {c1_dt:.1f} s'},
+        'cpu_baseline_cells': {'value': round(cc_rate, 2), 'unit': 'MC steps/s', 'cores': nth, 'kind': 'port',
+                               'cpu': model,
+                               'sample': f'oracle with a cell list of group centres (same inclusion sets), '
+                                         f'{nth} threads over the systems, chunks of 5 moves round-robin from '
+                                         f'move 0 (up to move {cc_pos}): {cc_done} MC steps in {cc_dt:.1f} s'},
     }
     print(json.dumps(out), flush=True)
 

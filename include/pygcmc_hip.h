@@ -120,6 +120,15 @@ int gcmc_engine_download(gcmc_engine *e, int replica, AtomArray *frag, residue *
  * (attempts/accepts: fragTypeNum*4 entries, index species*4 + kind). */
 int gcmc_engine_stats(gcmc_engine *e, int replica, double *energy, long long *attempts,
                       long long *accepts);
+/* Device copy of every replica's current state (positions, charges, types,
+ * cell list, species counts, step, statistics and counters), and its return:
+ * restore queues device-to-device copies on the engine's stream, so a chain
+ * can be rerun from the same start with inputs resident in HBM (the
+ * benchmark's passes).  The reference has no counterpart: its runGCMC_cuda
+ * (gcmc/cpp/gcmc.cu:21-118) copies the host arrays in on every call.
+ * Systems with Ewald electrostatics are refused. */
+int gcmc_engine_snapshot(gcmc_engine *e);
+int gcmc_engine_restore(gcmc_engine *e);
 /* Milliseconds of the last gcmc_engine_run measured with HIP events on the
  * engine's stream: the whole run, and the persistent kernel alone (batched
  * mode; 0 otherwise). */

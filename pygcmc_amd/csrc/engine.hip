@@ -113,6 +113,13 @@ struct HostRep {
     DevBuf trbuf;
     int maxk = 1;
     int host_step = 0;
+    size_t nmslot = 0;         // entries of mob_slot / mob_rec
+    // device snapshot of the state the chain changes (gcmc_engine_snapshot)
+    DevBuf snap;
+    float4 *s_res_pos = nullptr, *s_atom = nullptr, *s_mob_rec = nullptr;
+    int *s_atype = nullptr, *s_mob_slot = nullptr, *s_mob_loc = nullptr;
+    int2 *s_cell = nullptr;
+    int snap_step = 0;
     float ew_beta = 0.0f;      // > 0: Ewald electrostatics with PME
     float ew_spacing = 0.0f;
     gcmc_pme *pme = nullptr;
@@ -153,6 +160,7 @@ struct gcmc_engine {
     int nt = 256;              // threads per workgroup
     XcdMap xm{};
     int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]
+    State *d_state_snap = nullptr;  // State of every replica at gcmc_engine_snapshot
     double *d_adj_work = nullptr;  // column-sum partials of the adjoint backward
     size_t adj_work_n = 0;
 };
@@ -239,6 +247,12 @@ hipError_t put(T **dst, DevBuf &b, const T *src, size_t n)
     if (e != hipSuccess) return e;
     if (n) e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
     return e;
+}
+
+template <class T>
+hipError_t d2d(T *dst, const T *src, size_t n, hipStream_t s)
+{
+    return n ? hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s) : hipSuccess;
 }
 
 int upload_rep(gcmc_engine *E, HostRep &H)
@@ -431,6 +445,7 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     HIPX(put((FixRec **)&R.fix, B, frec.data(), frec.size()));
     HIPX(put(&R.mob_rec, B, mrec.data(), mrec.size()));
     HIPX(put(&R.mob_slot, B, mslot.data(), mslot.size()));
+    H.nmslot = mslot.size();
     HIPX(put(&R.mob_loc, B, mloc.data(), mloc.size()));
     HIPX(put((int **)&R.stencil, B, st.data(), st.size()));
     HIPX(B.alloc(&R.tc, H.maxk));
@@ -728,12 +743,14 @@ int gcmc_engine_destroy(gcmc_engine *E)
         if (H->pme) gcmc_pme_destroy(H->pme);
         H->buf.release();
         H->trbuf.release();
+        H->snap.release();
         delete H;
     }
     if (E->d_rep) hipFree(E->d_rep);
     if (E->d_state) hipFree(E->d_state);
     if (E->d_sched) hipFree(E->d_sched);
     if (E->d_cls) hipFree(E->d_cls);
+    if (E->d_state_snap) hipFree(E->d_state_snap);
     if (E->d_adj_work) hipFree(E->d_adj_work);
     if (E->ev0) hipEventDestroy(E->ev0);
     if (E->ev1) hipEventDestroy(E->ev1);
@@ -1026,6 +1043,59 @@ int gcmc_engine_stats(gcmc_engine *E, int r, double *energy, long long *attempts
     if (energy) *energy = S.energy;
     if (attempts) std::memcpy(attempts, S.att, sizeof(long long) * 4 * T);
     if (accepts) std::memcpy(accepts, S.acc, sizeof(long long) * 4 * T);
+    return 0;
+}
+
+int gcmc_engine_snapshot(gcmc_engine *E)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    for (HostRep *H : E->reps)
+        if (H->pme) return fail(GCMC_E_STATE, "snapshots of systems with Ewald electrostatics are not supported");
+    const int R = (int)E->reps.size();
+    if (!E->d_state_snap) HIPX(hipMalloc(&E->d_state_snap, R * sizeof(State)));
+    for (HostRep *Hp : E->reps) {
+        HostRep &H = *Hp;
+        const Rep &P = H.rep;
+        if (!H.s_res_pos) {
+            HIPX(H.snap.alloc(&H.s_res_pos, (size_t)P.nres));
+            HIPX(H.snap.alloc(&H.s_atom, (size_t)P.natom));
+            HIPX(H.snap.alloc(&H.s_atype, (size_t)P.natom));
+            HIPX(H.snap.alloc(&H.s_cell, (size_t)P.ncell + 1));
+            HIPX(H.snap.alloc(&H.s_mob_slot, H.nmslot));
+            HIPX(H.snap.alloc(&H.s_mob_rec, H.nmslot));
+            HIPX(H.snap.alloc(&H.s_mob_loc, (size_t)P.nres));
+        }
+        HIPX(d2d(H.s_res_pos, P.res_pos, (size_t)P.nres, E->stream));
+        HIPX(d2d(H.s_atom, P.atom, (size_t)P.natom, E->stream));
+        HIPX(d2d(H.s_atype, P.atype, (size_t)P.natom, E->stream));
+        HIPX(d2d(H.s_cell, P.cell, (size_t)P.ncell + 1, E->stream));
+        HIPX(d2d(H.s_mob_slot, P.mob_slot, H.nmslot, E->stream));
+        HIPX(d2d(H.s_mob_rec, P.mob_rec, H.nmslot, E->stream));
+        HIPX(d2d(H.s_mob_loc, P.mob_loc, (size_t)P.nres, E->stream));
+        H.snap_step = H.host_step;
+    }
+    HIPX(d2d(E->d_state_snap, E->d_state, (size_t)R, E->stream));
+    HIPX(hipStreamSynchronize(E->stream));
+    return 0;
+}
+
+int gcmc_engine_restore(gcmc_engine *E)
+{
+    if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
+    if (!E->d_state_snap) return fail(GCMC_E_STATE, "no snapshot taken");
+    for (HostRep *Hp : E->reps) {
+        HostRep &H = *Hp;
+        const Rep &P = H.rep;
+        HIPX(d2d(P.res_pos, H.s_res_pos, (size_t)P.nres, E->stream));
+        HIPX(d2d(P.atom, H.s_atom, (size_t)P.natom, E->stream));
+        HIPX(d2d(P.atype, H.s_atype, (size_t)P.natom, E->stream));
+        HIPX(d2d(P.cell, H.s_cell, (size_t)P.ncell + 1, E->stream));
+        HIPX(d2d(P.mob_slot, H.s_mob_slot, H.nmslot, E->stream));
+        HIPX(d2d(P.mob_rec, H.s_mob_rec, H.nmslot, E->stream));
+        HIPX(d2d(P.mob_loc, H.s_mob_loc, (size_t)P.nres, E->stream));
+        H.host_step = H.snap_step;
+    }
+    HIPX(d2d(E->d_state, E->d_state_snap, E->reps.size(), E->stream));
     return 0;
 }
 

@@ -77,6 +77,8 @@ def lib():
         L.gcmc_engine_step.argtypes = [P, I]
         L.gcmc_engine_download.argtypes = [P, I, P, P, P]
         L.gcmc_engine_stats.argtypes = [P, I, P, P, P]
+        L.gcmc_engine_snapshot.argtypes = [P]
+        L.gcmc_engine_restore.argtypes = [P]
         L.gcmc_engine_last_run_ms.argtypes = [P, P]
         L.gcmc_engine_last_kernel_ms.argtypes = [P, P]
         L.gcmc_engine_launch_info.argtypes = [P, P]
@@ -254,6 +256,16 @@ class Engine:
         acc = np.zeros((T, 4), np.int64)
         _check(lib().gcmc_engine_stats(self._h, r, ctypes.byref(e), _p(att), _p(acc)))
         return {'energy': e.value, 'attempts': att, 'accepts': acc}
+
+    def snapshot(self):
+        """Keep a device copy of every system's current state (see restore)."""
+        _check(lib().gcmc_engine_snapshot(self._h))
+
+    def restore(self):
+        """Return every system to the last snapshot (device-to-device copies
+        queued on the engine's stream): step, species counts, positions, cell
+        list, statistics and work counters."""
+        _check(lib().gcmc_engine_restore(self._h))
 
     def set_trace(self, r, max_steps, max_k):
         _check(lib().gcmc_engine_set_trace(self._h, r, int(max_steps), int(max_k)))
