@@ -220,6 +220,20 @@ int gcmc_engine_adjoint_backward(gcmc_engine *e, int npose, int per_pose, int nc
                                  const float *dEdx, const float *partial, const float *weights,
                                  float *grad_pose, float *grad_cols, void *stream);
 
+/* The same backward without any allocation: no counterpart in the reference
+ * (pyGCMC has no adjoint); for callers that cannot allocate device memory --
+ * code under HIP graph capture (a captured autograd step), tight serving
+ * loops, harnesses that forbid hipMalloc.  Identical arguments and results to
+ * gcmc_engine_adjoint_backward minus the engine; its column-sum workspace is
+ * a fixed per-device array inside the library (1 MiB of FP64, held until
+ * the process ends), so ncol must be at most 4096 (GCMC_E_ARG otherwise).
+ * Every call on one device shares that workspace: calls that overlap on a
+ * device (two streams, two threads) race.  DEVICE pointers, queued on
+ * `stream` (NULL = the default stream). */
+int gcmc_adjoint_backward_noalloc(int npose, int per_pose, int ncol, const float *dEdx,
+                                  const float *partial, const float *weights, float *grad_pose,
+                                  float *grad_cols, void *stream);
+
 /* ---- Ewald electrostatics (no counterpart in the reference) ---------------
  * Before upload: give replica `replica` Ewald electrostatics with splitting
  * beta (1/A) and a PME grid of about grid_spacing A (order-4 B-splines).

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1353,6 +1354,72 @@ int gcmc_engine_adjoint_backward(gcmc_engine *E, int npose, int per_pose, int nc
         hipLaunchKernelGGL(k_adjoint_colsum, dim3((ncol + 63) / 64, NCH), dim3(1024), 0, s, npose, ncol, partial,
                            weights, E->d_adj_work);
         hipLaunchKernelGGL(k_adjoint_chunksum, dim3((ncol + 255) / 256), dim3(256), 0, s, NCH, ncol, E->d_adj_work,
+                           grad_cols);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GCMC_E_HIP, std::string("adjoint backward: ") + hipGetErrorString(e));
+    return 0;
+}
+
+}  // extern "C"
+
+// Workspace of gcmc_adjoint_backward_noalloc: FP64 column partial sums, one
+// row of at most ADJ_NOALLOC_MAX_COLS per pose chunk.  One copy per device,
+// held until the process ends; every call on a device shares it (each call
+// writes all the entries it then reads, so nothing needs clearing; calls
+// that overlap on one device race).
+namespace {
+constexpr int ADJ_NOALLOC_NCH = 32;
+constexpr int ADJ_NOALLOC_MAX_COLS = 4096;
+constexpr int ADJ_NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[ADJ_NOALLOC_NCH * ADJ_NOALLOC_MAX_COLS];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[ADJ_NOALLOC_MAX_DEVICES] = {};
+
+// Device address of the current device's `scratch`, looked up once per device
+// with that device current (the array's name in host code is only a host-side
+// stand-in, not a device address).
+int scratch_address(double **out)
+{
+    int dev = -1;
+    HIPX(hipGetDevice(&dev));
+    if (dev < 0 || dev >= ADJ_NOALLOC_MAX_DEVICES) return fail(GCMC_E_DEVICE, "device index beyond the workspace table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GCMC_E_HIP, "hipGetSymbolAddress returned a null workspace address");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int gcmc_adjoint_backward_noalloc(int npose, int per_pose, int ncol, const float *dEdx, const float *partial,
+                                  const float *weights, float *grad_pose, float *grad_cols, void *stream)
+{
+    if (npose < 0 || per_pose < 0 || ncol < 0) return fail(GCMC_E_ARG, "negative size");
+    if (ncol > ADJ_NOALLOC_MAX_COLS)
+        return fail(GCMC_E_ARG, "ncol " + std::to_string(ncol) + " exceeds the fixed workspace (at most " +
+                                    std::to_string(ADJ_NOALLOC_MAX_COLS) + " columns)");
+    if ((npose > 0 && per_pose > 0 && (!dEdx || !grad_pose)) || (ncol > 0 && (!partial || !grad_cols)))
+        return fail(GCMC_E_ARG, "null array");
+    double *work = nullptr;
+    int rc = scratch_address(&work);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nx = (size_t)npose * per_pose;
+    if (nx > 0)
+        hipLaunchKernelGGL(k_scale_rows, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, npose, per_pose, dEdx,
+                           weights, grad_pose);
+    if (ncol > 0) {
+        hipLaunchKernelGGL(k_adjoint_colsum, dim3((ncol + 63) / 64, ADJ_NOALLOC_NCH), dim3(1024), 0, s, npose, ncol,
+                           partial, weights, work);
+        hipLaunchKernelGGL(k_adjoint_chunksum, dim3((ncol + 255) / 256), dim3(256), 0, s, ADJ_NOALLOC_NCH, ncol, work,
                            grad_cols);
     }
     const hipError_t e = hipGetLastError();
