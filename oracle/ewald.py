@@ -166,6 +166,87 @@ class EwaldEnv:
                                                                 self.S_im[None, :] * np.sin(pp))).sum(1)
 
 
+def _spline_points(x, L, K):
+    """Grid points and order-4 cardinal B-spline weights of positions x along
+    one axis (scaled coordinate u = K frac(x / L), points floor(u) - i with
+    weight M4(u - floor(u) + i), i = 0..3), FP64."""
+    fr = np.asarray(x, np.float64) / L
+    u = (fr - np.floor(fr)) * K
+    u = np.where(u >= K, 0.0, u)
+    f = np.floor(u)
+    t = u - f
+    idx = (f.astype(np.int64)[:, None] - np.arange(4)[None, :]) % K
+    w = bspline4(t[:, None] + np.arange(4)[None, :])
+    return idx, w
+
+
+class PMEEnv:
+    """Smooth particle-mesh Ewald (Essmann et al. 1995, order-4 B-splines) in
+    FP64: the same discretisation as the GPU's PME kernels (charge grid Q,
+    influence function G with exact B-spline moduli, potential grid
+    Phi = IFFT(G FFT(Q)) unnormalised), so that a comparison with them
+    measures arithmetic error alone; its discretisation error against the
+    Ewald sums (EwaldEnv) is pinned separately."""
+
+    def __init__(self, xyz, q, L, beta, K):
+        xyz = np.asarray(xyz, np.float64).reshape(-1, 3)
+        q = np.asarray(q, np.float64)
+        self.L = np.asarray(L, np.float64)
+        self.K = tuple(int(k) for k in K)
+        K0, K1, K2 = self.K
+        Q = np.zeros(K0 * K1 * K2)
+        i0, w0 = _spline_points(xyz[:, 0], self.L[0], K0)
+        i1, w1 = _spline_points(xyz[:, 1], self.L[1], K1)
+        i2, w2 = _spline_points(xyz[:, 2], self.L[2], K2)
+        for a in range(4):
+            for b in range(4):
+                flat = (i0[:, a, None] * K1 + i1[:, b, None]) * K2 + i2
+                Q += np.bincount(flat.ravel(), weights=(q[:, None] * w0[:, a, None] * w1[:, b, None] *
+                                                        w2).ravel(), minlength=len(Q))
+        Q = Q.reshape(self.K)
+        V = float(np.prod(self.L))
+
+        def freq(n, Ln, half=False):
+            m = np.arange(n // 2 + 1 if half else n)
+            if not half:
+                m = np.where(m <= n // 2, m, m - n)
+            return m / Ln
+
+        def bmod2(n, count):
+            m = np.arange(count)
+            k = np.arange(3)
+            M4 = np.array([1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0])
+            ph = 2.0 * math.pi * m[:, None] * k[None, :] / n
+            re = (M4 * np.cos(ph)).sum(1)
+            im = (M4 * np.sin(ph)).sum(1)
+            return 1.0 / (re * re + im * im)
+
+        ma = freq(K0, self.L[0])[:, None, None]
+        mb = freq(K1, self.L[1])[None, :, None]
+        mc = freq(K2, self.L[2], half=True)[None, None, :]
+        m2 = ma * ma + mb * mb + mc * mc
+        B = bmod2(K0, K0)[:, None, None] * bmod2(K1, K1)[None, :, None] * bmod2(K2, K2 // 2 + 1)[None, None, :]
+        with np.errstate(divide='ignore', invalid='ignore'):
+            G = KCOUL / (math.pi * V) * np.exp(-math.pi ** 2 * m2 / beta ** 2) / m2 * B
+        G[0, 0, 0] = 0.0
+        X = np.fft.rfftn(Q)
+        self.energy = 0.5 * float((G * np.abs(X) ** 2 * np.where(
+            (np.arange(K2 // 2 + 1) == 0) | (2 * np.arange(K2 // 2 + 1) == K2), 1.0, 2.0)[None, None, :]).sum())
+        self.Phi = np.fft.irfftn(G * X, s=self.K, axes=(0, 1, 2)) * (K0 * K1 * K2)
+
+    def phi(self, points):
+        pts = np.asarray(points, np.float64).reshape(-1, 3)
+        K0, K1, K2 = self.K
+        i0, w0 = _spline_points(pts[:, 0], self.L[0], K0)
+        i1, w1 = _spline_points(pts[:, 1], self.L[1], K1)
+        i2, w2 = _spline_points(pts[:, 2], self.L[2], K2)
+        out = np.zeros(len(pts))
+        for a in range(4):
+            for b in range(4):
+                out += w0[:, a] * w1[:, b] * (w2 * self.Phi[i0[:, a, None], i1[:, b, None], i2]).sum(1)
+        return out
+
+
 def present_groups(sysd, exclude=-1):
     """Residue indices in the state: fixed groups, then each species' active
     slots (calcEnergy's order, gcmc/cpp/gcmc_energy.h:152-191), minus exclude."""

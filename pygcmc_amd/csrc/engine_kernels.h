@@ -1748,10 +1748,12 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             advance_and_publish<NT>(C, S, sch, L, true, mc.step, mc.spi);
             if (tid == 0) {
                 const unsigned long long ta2 = wall_clock64();
-                // timers after the next move is published, off its path
-                S.t_step += ta0 - tpub;
-                S.t_accept += ta1 - ta0;
-                S.t_prep += ta2 - ta1;
+                // timers after the next move is published, off its path; the
+                // workgroup that finishes that move may already be adding its
+                // own, so these are atomic adds too
+                count_add<true>(&S.t_step, ta0 - tpub);
+                count_add<true>(&S.t_accept, ta1 - ta0);
+                count_add<true>(&S.t_prep, ta2 - ta1);
                 int row;
                 if (tracing(C, step0, row)) {
                     C.tr.tick[4 * row + 0] = tpub;

@@ -48,6 +48,9 @@ class OccupancyMap:
         self.conc = float(frag['conc'])
         self.samples = 0
         self._counts = torch.zeros(int(np.prod(self.shape)), dtype=torch.int32, device='cuda')
+        # the zero-fill runs on torch's stream, the counting kernel on the
+        # engine's: wait for the fill before any accumulate() is queued
+        torch.cuda.current_stream().synchronize()
         self._host = None
 
     def accumulate(self):
@@ -74,6 +77,7 @@ class OccupancyMap:
     def reset(self):
         self.eng.synchronize()
         self._counts.zero_()
+        torch.cuda.current_stream().synchronize()
         self.samples = 0
 
     def free_energy(self, cap=None):

@@ -156,3 +156,22 @@ def block_stats(x, nblocks=40):
     v = ((b - b.mean()) ** 2).mean(1)
     return float(m.mean()), float(m.std(ddof=1) / math.sqrt(nblocks)), float(v.mean()), \
         float(v.std(ddof=1) / math.sqrt(nblocks))
+
+
+def interacting_particles(seed=13, box=18.0, conc=30.0, muex=1.0, cutoff=9.0, mcsteps=200000, n_sites=20):
+    """One-atom LJ particles that interact with each other and with fixed LJ
+    sites (a dense-ish LJ fluid in a field): no closed form, so its mean
+    count is compared between independent chains.  Insertions, deletions
+    and translations (the schedule's own move mix)."""
+    s = field_particles(seed=seed, box=box, n_sites=n_sites, conc=conc, muex=muex, cutoff=cutoff,
+                        mcsteps=mcsteps)
+    rng = np.random.default_rng(seed + 1)
+    s['moves'] = rng.choice(np.array([0, 1, 2], np.int32), mcsteps, p=[0.35, 0.35, 0.3]).astype(np.int32)
+    # restore the particle-particle pair: geometric / arithmetic combination
+    # of the particle's own parameters as the other pairs
+    Y = int(s['info'][0]['ffYNum'])
+    t = s['types2'].index('LJP')
+    sig, eps = dataset.charmm_to_gromacs(0.2, 1.6)
+    s['ff'][2 * (t * Y + t)] = sig
+    s['ff'][2 * (t * Y + t) + 1] = eps
+    return s

@@ -52,3 +52,30 @@ def test_bspline_partition_of_unity():
     m1 = sum((u + k) * ewald.bspline4(u + k) for k in range(4))
     assert np.allclose(m1, 2.0, atol=1e-13)
     assert math.isclose(float(ewald.bspline4(np.array([2.0]))[0]), 2.0 / 3.0)
+
+
+def test_fp64_pme_converges_to_ewald_sums():
+    """The FP64 SPME restatement against the Ewald sums: the reciprocal
+    energy and potential converge as h^4 (order-4 B-splines) -- halving the
+    grid spacing cuts the error ~16x -- and E_rec = 1/2 sum q phi."""
+    rng = np.random.default_rng(5)
+    L = np.array([18.0, 16.0, 20.0])
+    xyz = rng.random((60, 3)) * L
+    q = rng.normal(0, 0.5, 60)
+    q -= q.mean()
+    beta = 0.34
+    kmax = tuple(int(np.ceil(4.8 * l * beta / np.pi)) for l in L)
+    e_ref = ewald.recip_energy(xyz, q, L, beta, kmax)
+    pts = rng.random((40, 3)) * L
+    env = ewald.EwaldEnv(xyz, q, L, beta, kmax)
+    phi_ref = env.phi(pts)
+    errs = []
+    for h in (1.0, 0.5, 0.25):
+        K = [int(round(l / h)) for l in L]
+        pme = ewald.PMEEnv(xyz, q, L, beta, K)
+        assert abs(0.5 * float((q * pme.phi(xyz)).sum()) - pme.energy) <= 1e-9 * abs(pme.energy)
+        errs.append((abs(pme.energy - e_ref), float(np.abs(pme.phi(pts) - phi_ref).max())))
+    print('PME FP64 vs Ewald (energy, max phi error) at h = 1, 0.5, 0.25 A:', errs)
+    for a, b in zip(errs, errs[1:]):
+        assert b[1] < a[1] / 8 and b[0] < a[0] / 8     # h^4: ~16x per halving
+    assert errs[-1][1] < 1e-3 and errs[-1][0] < 1e-2
