@@ -339,6 +339,36 @@ def adjoint_bench(npose, reps=5):
             'env_atoms': env, 'dtype': 'fp32 (energies fp64 from compensated fp32)'}
 
 
+def ewald_bench(moves, warm=32):
+    """The 300,000-atom slab (150 x 150 x 120 A, 15 A pore, water +
+    methylammonium / acetate, PME 1 A grid, beta 0.34 /A, 12 A cutoff) in
+    Ewald mode: `moves` moves after `warm` untimed ones, in the plain form
+    (launches issued move by move) and the graph form (captured 32-move
+    chunks replayed); both compute the same chain."""
+    from pygcmc_amd import synth
+    from pygcmc_amd.engine import Engine
+    s = synth.slab(mcsteps=50000)
+    last = s['res'][int(s['frag'][0]['startRes']) - 1]
+    out = {'fixed_atoms': int(last['atomStart'] + last['atomNum']), 'moves': moves}
+    states = {}
+    for mode in ('plain', 'graph'):
+        g = copies([s])[0]
+        eng = Engine()
+        eng.add_system(g)
+        eng.set_ewald(0, 0.34, 1.0)
+        eng.upload()
+        eng.run(warm, mode)
+        ms = eng.run(moves, mode)
+        eng.download(0)
+        states[mode] = (g['atoms'].tobytes(), eng.stats(0)['energy'])
+        out[f'{mode}_ms'] = round(ms, 2)
+        out[f'{mode}_steps_per_s'] = round(moves / (ms / 1e3), 1)
+        out['molecules_after'] = int(g['frag']['totalNum'].sum())
+        eng.close()
+    out['same_chain'] = states['plain'] == states['graph']
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -364,6 +394,8 @@ def main():
                     help='only the timed passes (for profiler runs): no graph, plain, lock-step, '
                          'adjoint, set-up or CPU legs')
     ap.add_argument('--adjoint-poses', type=int, default=4096)
+    ap.add_argument('--ewald-moves', type=int, default=256,
+                    help='moves of the 300k-atom PME slab timed in the plain and graph forms (0: skip)')
     ap.add_argument('--setup-bench', type=int, default=1,
                     help='time set-up of a 4wp7-sized system from PDB/TOP/force-field files')
     ap.add_argument('--autotune', type=int, default=0, help='pick cell/workgroup sizes by timing short runs')
@@ -489,6 +521,9 @@ def main():
     adj = adjoint_bench(args.adjoint_poses) if args.adjoint_poses > 0 else None
     if adj:
         log(f"adjoint: {adj['poses']} poses fwd+bwd in {adj['ms']:.2f} ms")
+    ew = ewald_bench(args.ewald_moves) if args.ewald_moves > 0 else None
+    if ew:
+        log(f'ewald slab: {ew}')
     setup = setup_bench() if args.setup_bench else None
     if setup:
         log(f"set-up from files: {setup}")
@@ -536,6 +571,7 @@ def main():
         'tuning': tuning,
         'autotune_ms': None if tune_table is None else [[t, round(ms, 3)] for t, ms in tune_table],
         'adjoint': adj,
+        'ewald_slab': ew,
         'setup_from_files': setup,
         'roofline': {'bound': 'hbm', 'kernel': 'k_batched',
                      'achieved': round(gbs, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

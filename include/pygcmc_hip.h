@@ -93,6 +93,10 @@ typedef struct gcmc_tuning {
     int wg_size;         /* threads per workgroup: 256, 512 or 1024; 0 = 256 */
     int fp32_coulomb;    /* 1 = Coulomb pair terms in FP32 like the LJ terms; 0 (default) =
                             Coulomb in FP64 from the exactly formed FP32 displacement */
+    int claim_affinity;  /* batched form: 0 = claim the move with the fewest unclaimed items
+                            first; N > 0 = the same, except that a move of at least N unclaimed
+                            items of the system whose tables a workgroup holds in LDS comes
+                            before other such moves.  Changes the order of work, never results */
 } gcmc_tuning;
 
 #define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */
@@ -240,8 +244,11 @@ int gcmc_adjoint_backward_noalloc(int npose, int per_pose, int ncol, const float
  * Its trial energies then use K q q' erfc(beta r)/r in the group-cutoff pair
  * terms plus the reciprocal interaction sum_f q_f phi(x_f) with the PME
  * potential phi of the current state (the moving molecule's own share
- * removed), and the potential is updated after every move.  Such replicas
- * run in GCMC_MODE_PLAIN only.  The reference ignores InfoStruct.PME, and so
+ * removed), and the potential is updated after every move.  In the batched
+ * and graph forms an engine's systems are either all Ewald systems or none:
+ * Ewald systems then run move by move (each move's trials in one launch,
+ * accept, PME update), the graph form replaying a captured chunk of moves;
+ * the chains equal the plain form's.  The reference ignores InfoStruct.PME, and so
  * do runGCMC_hip and gcmc_engine_add_system.  eval_poses adds the same
  * reciprocal term; its exclude[i], when a molecule of the evaluated species,
  * must be one present in the state. */

@@ -157,6 +157,10 @@ struct gcmc_engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_nsteps = -1;
+    // Ewald systems, graph form: a captured chunk of ew_graph_moves moves
+    hipGraph_t ew_graph = nullptr;
+    hipGraphExec_t ew_gexec = nullptr;
+    int ew_graph_moves = 0;
     int spin_limit = 1 << 22;  // polls without progress anywhere (~4 s)
     int nt = 256;              // threads per workgroup
     XcdMap xm{};
@@ -249,6 +253,9 @@ hipError_t put(T **dst, DevBuf &b, const T *src, size_t n)
     if (n) e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
     return e;
 }
+
+// moves of the captured chunk of the graph form for Ewald systems
+constexpr int EW_GRAPH_MOVES = 32;
 
 template <class T>
 hipError_t d2d(T *dst, const T *src, size_t n, hipStream_t s)
@@ -361,12 +368,18 @@ int upload_rep(gcmc_engine *E, HostRep &H)
         double nbar = (double)I.cryst[0] * (double)I.cryst[1] * (double)I.cryst[2] *
                       (double)F.conc * 0.0006023;
         S.B = BETA * (double)F.muex + std::log(nbar);
+        int utype[MAXF];
         for (int a = 0; a < F.num_atoms; ++a) {
             const Atom &A = F.atoms[a];
             S.tmpl[a] = make_float4(A.position[0], A.position[1], A.position[2], A.charge);
             S.kq[a] = (float)(KCOUL * (double)A.charge);
             S.trow[a] = A.type * R.Y;
             S.ttype[a] = A.type;
+            // distinct types in order of first appearance: utype[u] is type u's id
+            int u = 0;
+            while (u < S.nu && utype[u] != A.type) ++u;
+            if (u == S.nu) utype[S.nu++] = A.type;
+            S.urow[a] = u * R.Y;
             if (A.type < 0 || A.type >= R.X)
                 return fail(GCMC_E_ARG, "fragment atom type outside the pair table rows");
         }
@@ -429,13 +442,14 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     HIPX(put((float4 **)&R.pair, B, pair.data(), pair.size()));
     HIPX(put((Species **)&R.sp, B, sp.data(), sp.size()));
     {
-        // each species' rows of the pair table, atom by atom, contiguous
-        // (staging them in LDS then needs only the species index)
+        // each species' rows of the pair table, one per distinct atom type in
+        // order of first appearance, contiguous (staging them in LDS then
+        // needs only the species index; atoms of one type share a row)
         std::vector<float4> spair((size_t)T * MAXF * R.Y, make_float4(0.f, 0.f, 0.f, 0.f));
         for (int s = 0; s < T; ++s)
             for (int a = 0; a < sp[s].nf; ++a)
                 for (int y = 0; y < R.Y; ++y)
-                    spair[((size_t)s * MAXF + a) * R.Y + y] = pair[(size_t)sp[s].trow[a] + y];
+                    spair[((size_t)s * MAXF * R.Y + sp[s].urow[a]) + y] = pair[(size_t)sp[s].trow[a] + y];
         HIPX(put((float4 **)&R.spair, B, spair.data(), spair.size()));
     }
     {
@@ -496,11 +510,20 @@ int upload_rep(gcmc_engine *E, HostRep &H)
 
 int compute_layout(gcmc_engine *E)
 {
-    // staged pair table: the rows of one species' atoms (nf x Y entries)
+    // staged pair table: the rows of one species' distinct atom types (nu x Y entries)
     int nst = 1, fy = 1;
     for (HostRep *H : E->reps) {
         nst = std::max(nst, H->rep.nst);
-        for (const AtomArray &F : H->frag) fy = std::max(fy, F.num_atoms * H->rep.Y);
+        for (const AtomArray &F : H->frag) {
+            // distinct atom types of the species: its staged rows
+            int nu = 0, seen[MAXF];
+            for (int a = 0; a < F.num_atoms; ++a) {
+                int u = 0;
+                while (u < nu && seen[u] != F.atoms[a].type) ++u;
+                if (u == nu) seen[nu++] = F.atoms[a].type;
+            }
+            fy = std::max(fy, nu * H->rep.Y);
+        }
     }
     const int pair_bytes = 16 * fy;
     E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
@@ -584,7 +607,7 @@ void launch_batched(gcmc_engine *E, int nsteps)
     const int nwg = std::min(E->nwg, E->resident);
     if (E->timed_kernel) (void)hipEventRecord(E->evk0, E->stream);
     hipLaunchKernelGGL((k_batched<NT, S>), dim3(nwg), dim3(NT), lds, E->stream, E->d_rep, E->d_state, R,
-                       E->d_sched, E->lay, E->spin_limit, E->xm);
+                       E->d_sched, E->lay, E->spin_limit, E->xm, std::min(std::max(E->tune.claim_affinity, 0), 254));
     if (E->timed_kernel) (void)hipEventRecord(E->evk1, E->stream);
 }
 
@@ -614,6 +637,35 @@ void launch_plain(gcmc_engine *E, int nsteps)
     }
 }
 
+// Systems with Ewald electrostatics in the batched and graph forms: move by
+// move, every system's trial evaluations in one launch, its accept + prep,
+// then the PME update (the accepted charge changes spread into the charge
+// grid, FFT solve for the new potential) -- the reciprocal potential has to
+// be current before the next move's evaluations read it.  The same kernels
+// in the same order as the plain form, so the chains are identical.
+template <int NT, bool S>
+void launch_ewald_begin(gcmc_engine *E, int nsteps)
+{
+    const int R = (int)E->reps.size();
+    hipLaunchKernelGGL(k_begin, dim3((R + 255) / 256), dim3(256), 0, E->stream, E->d_state, E->d_rep, 0, R, nsteps,
+                       E->d_sched);
+    hipLaunchKernelGGL((k_start<NT, S>), dim3(R), dim3(NT), E->lay.total, E->stream, E->d_rep, E->d_state,
+                       E->d_sched, E->lay, 0, 0);
+}
+
+template <int NT, bool S>
+void launch_ewald_moves(gcmc_engine *E, int nmoves)
+{
+    for (int m = 0; m < nmoves; ++m)
+        for (int r = 0; r < (int)E->reps.size(); ++r) {
+            HostRep &H = *E->reps[r];
+            launch_plain_step<NT, S, true>(E, r);
+            gcmc_internal::pme_spread_counted(H.pme, H.rep.pme_cap, H.rep.pme_ndelta, H.rep.pme_delta,
+                                              H.rep.pme_ndelta, E->stream);
+            gcmc_pme_solve(H.pme, nullptr, E->stream);
+        }
+}
+
 // dispatch on (workgroup size, pair table in LDS)
 template <template <int, bool> class F, class... A>
 void dispatch(gcmc_engine *E, A... args)
@@ -629,6 +681,14 @@ struct RunPlain {
 template <int NT, bool S>
 struct RunBatched {
     static void run(gcmc_engine *E, int n) { launch_batched<NT, S>(E, n); }
+};
+template <int NT, bool S>
+struct RunEwaldBegin {
+    static void run(gcmc_engine *E, int n) { launch_ewald_begin<NT, S>(E, n); }
+};
+template <int NT, bool S>
+struct RunEwaldMoves {
+    static void run(gcmc_engine *E, int n) { launch_ewald_moves<NT, S>(E, n); }
 };
 template <int NT, bool S>
 struct RunPoses {
@@ -699,6 +759,13 @@ int sync_rep_struct(gcmc_engine *E, int r)
         E->graph = nullptr;
         E->graph_nsteps = -1;
     }
+    if (E->ew_gexec) {
+        hipGraphExecDestroy(E->ew_gexec);
+        hipGraphDestroy(E->ew_graph);
+        E->ew_gexec = nullptr;
+        E->ew_graph = nullptr;
+        E->ew_graph_moves = 0;
+    }
     return 0;
 }
 
@@ -740,6 +807,8 @@ int gcmc_engine_destroy(gcmc_engine *E)
     if (E->stream) hipStreamSynchronize(E->stream);
     if (E->gexec) hipGraphExecDestroy(E->gexec);
     if (E->graph) hipGraphDestroy(E->graph);
+    if (E->ew_gexec) hipGraphExecDestroy(E->ew_gexec);
+    if (E->ew_graph) hipGraphDestroy(E->ew_graph);
     for (HostRep *H : E->reps) {
         if (H->pme) gcmc_pme_destroy(H->pme);
         H->buf.release();
@@ -868,11 +937,33 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
     if (!E || !E->uploaded) return fail(GCMC_E_STATE, "engine not uploaded");
     if (nsteps < 0) return fail(GCMC_E_ARG, "negative step count");
     if (nsteps == 0) return 0;
-    if (mode != GCMC_MODE_PLAIN)
-        for (HostRep *H : E->reps)
-            if (H->pme) return fail(GCMC_E_ARG, "systems with Ewald electrostatics run in plain mode");
+    int n_ew = 0;
+    for (HostRep *H : E->reps) n_ew += H->pme ? 1 : 0;
+    if (mode != GCMC_MODE_PLAIN && n_ew > 0 && n_ew < (int)E->reps.size())
+        return fail(GCMC_E_ARG, "the batched and graph forms run either all systems with Ewald electrostatics "
+                                "or none; run mixed engines in plain mode");
+    if (mode != GCMC_MODE_PLAIN && mode != GCMC_MODE_BATCHED && mode != GCMC_MODE_GRAPH)
+        return fail(GCMC_E_ARG, "unknown run mode");
     HIPX(hipEventRecord(E->ev0, E->stream));
-    if (mode == GCMC_MODE_PLAIN) {
+    if (mode != GCMC_MODE_PLAIN && n_ew > 0) {
+        // moves this call runs: the most any system has left
+        int n = 0;
+        for (HostRep *H : E->reps) n = std::max(n, std::min(nsteps, std::max(H->rep.nmoves - H->host_step, 0)));
+        dispatch<RunEwaldBegin>(E, nsteps);
+        int done = 0;
+        if (mode == GCMC_MODE_GRAPH && n >= EW_GRAPH_MOVES) {
+            if (!E->ew_gexec) {
+                HIPX(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+                dispatch<RunEwaldMoves>(E, EW_GRAPH_MOVES);
+                HIPX(hipStreamEndCapture(E->stream, &E->ew_graph));
+                HIPX(hipGraphInstantiate(&E->ew_gexec, E->ew_graph, nullptr, nullptr, 0));
+                E->ew_graph_moves = EW_GRAPH_MOVES;
+            }
+            for (; done + E->ew_graph_moves <= n; done += E->ew_graph_moves)
+                HIPX(hipGraphLaunch(E->ew_gexec, E->stream));
+        }
+        if (done < n) dispatch<RunEwaldMoves>(E, n - done);
+    } else if (mode == GCMC_MODE_PLAIN) {
         dispatch<RunPlain>(E, nsteps);
     } else if (mode == GCMC_MODE_BATCHED) {
         E->timed_kernel = true;
@@ -892,8 +983,6 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
             E->graph_nsteps = nsteps;
         }
         HIPX(hipGraphLaunch(E->gexec, E->stream));
-    } else {
-        return fail(GCMC_E_ARG, "unknown run mode");
     }
     HIPX(hipGetLastError());
     HIPX(hipEventRecord(E->ev1, E->stream));
@@ -902,7 +991,7 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
     HIPX(hipEventElapsedTime(&ms, E->ev0, E->ev1));
     E->last_ms = ms;
     E->kernel_ms = 0.0;
-    if (mode == GCMC_MODE_BATCHED) {
+    if (mode == GCMC_MODE_BATCHED && n_ew == 0) {
         HIPX(hipEventElapsedTime(&ms, E->evk0, E->evk1));
         E->kernel_ms = ms;
     }

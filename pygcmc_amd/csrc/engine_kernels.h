@@ -431,13 +431,13 @@ __device__ inline void load_species_table(const Rep &C, const Lds &L)
     }
 }
 
-// pair-table rows of species spi's atoms into LDS, atom f's row at f * Y
-// (L.frow[f] = f * Y when staged)
+// pair-table rows of species spi's distinct atom types into LDS (atom f's
+// row at L.frow[f] = F.urow[f] when staged)
 template <int NT>
-__device__ inline void stage_species_rows(const Rep &C, int spi, int nf, const Lds &L)
+__device__ inline void stage_species_rows(const Rep &C, int spi, int nu, const Lds &L)
 {
     const auto SP = gbl(C.spair) + (size_t)spi * MAXF * C.Y;
-    const int n = nf * C.Y;
+    const int n = nu * C.Y;
     for (int i = threadIdx.x; i < n; i += NT) L.pair[i] = SP[i];
 }
 
@@ -729,7 +729,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
     // pass (eval_energy_pass)
     const bool stg = STAGED && stage_rows;
     if (stg) {
-        const int nrow = nf * C.Y;
+        const int nrow = F.nu * C.Y;
         const auto SP = gbl(C.spair) + (size_t)spi * MAXF * C.Y;
         for (int b = wave * 64; b < nrow; b += NT)
             if (b + lane < nrow) __builtin_amdgcn_global_load_lds((const void *)(SP + b + lane), (void *)&L.pair[b], 16, 0, 0);
@@ -794,7 +794,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
             }
         }
         L.frag[a] = make_float4(px, py, pz, F.kq[a]);
-        L.frow[a] = STAGED ? a * C.Y : F.trow[a];
+        L.frow[a] = STAGED ? F.urow[a] : F.trow[a];
         if (kind == K_INS || phase == 1) gbl(C.pose)[t * MAXF + a] = make_float4(px, py, pz, 0.f);
     }
     __syncthreads();
@@ -1616,7 +1616,7 @@ __device__ inline int my_xcc() { return (int)(__builtin_amdgcn_s_getreg((20) | (
 // (then sch->abort is set).
 template <int NT, bool STAGED>
 __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State *st, int R, Sched *sch,
-                                               LdsLayout lay, int spin_limit, XcdMap xm)
+                                               LdsLayout lay, int spin_limit, XcdMap xm, int aff)
 {
     extern __shared__ __align__(16) char smem[];
     const Lds L = carve(smem, lay);
@@ -1669,7 +1669,13 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
                     // Ties (and moves over 255 items) by claim_prio: the most
                     // estimated work left first.
                     const unsigned long long unc = ((w >> 20) & 0xFFFFFull) - (w & 0xFFFFFull);
-                    const unsigned size = unc > 255ull ? 255u : (unsigned)unc;
+                    unsigned size = unc > 255ull ? 255u : (unsigned)unc;
+                    // affinity (aff > 0): a move of at least aff unclaimed items
+                    // of the replica whose tables this workgroup holds in LDS
+                    // comes after the smaller moves and before the other large
+                    // ones (its species table and staged pair rows are reused)
+                    if (aff > 0 && size >= (unsigned)aff)
+                        size = (r == cached_rep) ? (unsigned)aff : (size < 255u ? size + 1u : 255u);
                     const unsigned prio = (unsigned)(w >> 40) >> 8;
                     while (__ballot(avail) != 0ull) {
                         const unsigned key =
@@ -1787,12 +1793,12 @@ __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int s
     const Rep &C = reps[r];
     const Species &F = C.sp[spi];
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x;
-    if (STAGED) stage_species_rows<NT>(C, spi, nf, L);
+    if (STAGED) stage_species_rows<NT>(C, spi, F.nu, L);
     load_species_table(C, L);
     if (tid < nf) {
         const float *p = poses + ((size_t)i * nf + tid) * 3;
         L.frag[tid] = make_float4(p[0], p[1], p[2], F.kq[tid]);
-        L.frow[tid] = STAGED ? tid * C.Y : F.trow[tid];
+        L.frow[tid] = STAGED ? F.urow[tid] : F.trow[tid];
     }
     __syncthreads();
     double e;

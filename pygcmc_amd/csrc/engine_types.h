@@ -44,6 +44,9 @@ struct Species {
     float kq[MAXF];        // 1388.431112 * charge, rounded once
     int trow[MAXF];        // type * Y: row of the pair table
     int ttype[MAXF];       // type
+    int urow[MAXF];        // u * Y: row of the atom's type in the species' staged rows
+    int nu;                // distinct atom types of the species (rows staged in LDS)
+    int pad2[3];
 };
 
 struct DevTrace {
@@ -87,7 +90,7 @@ struct Rep {
     float4 *atom;
     int *atype;
     const float4 *pair;      // (100 sigma^2 hi, 4 eps, 100 sigma^2 lo, 0) per (fragment type, type)
-    const float4 *spair;     // [T][MAXF][Y]: pair rows of each species' atoms, contiguous
+    const float4 *spair;     // [T][MAXF][Y]: pair rows of each species' distinct atom types, contiguous
     const Species *sp;
     // per cell c in [0, ncell]: (first fixed record, mobile count); the fixed
     // records of cell c are [cell[c].x, cell[c + 1].x), and cell[ncell].y

@@ -34,7 +34,7 @@ _lib = None
 class Tuning(ctypes.Structure):
     _fields_ = [('cell_size', ctypes.c_float), ('persistent_wgs', ctypes.c_int),
                 ('bucket_cap', ctypes.c_int), ('pair_lds', ctypes.c_int), ('wg_size', ctypes.c_int),
-                ('fp32_coulomb', ctypes.c_int)]
+                ('fp32_coulomb', ctypes.c_int), ('claim_affinity', ctypes.c_int)]
 
 
 class CTrace(ctypes.Structure):
@@ -139,9 +139,10 @@ def host_stencil_covers(centres, groups, cryst, startxyz, cutoff, cell_size=0.0)
 class Engine:
     """Independent GCMC systems on one GPU (see module docstring)."""
 
-    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=0, wg_size=0, fp32_coulomb=0):
+    def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=0, wg_size=0, fp32_coulomb=0,
+                 claim_affinity=0):
         self._h = ctypes.c_void_p()
-        t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds, wg_size, fp32_coulomb)
+        t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds, wg_size, fp32_coulomb, claim_affinity)
         _check(lib().gcmc_engine_create(ctypes.byref(self._h), ctypes.byref(t)))
         self.systems = []
         self._traces = {}
@@ -181,7 +182,8 @@ class Engine:
 
     def set_ewald(self, r, beta=0.34, grid_spacing=1.0):
         """Ewald electrostatics for replica r (before upload): real-space
-        erfc pair terms and PME reciprocal potential; plain mode only."""
+        erfc pair terms and PME reciprocal potential.  The batched and graph
+        forms need every system of the engine in Ewald mode (or none)."""
         _check(lib().gcmc_engine_set_ewald(self._h, int(r), float(beta), float(grid_spacing)))
 
     def ewald_potential(self, r, xyz):

@@ -12,8 +12,9 @@ FP64 oracle, with the error printed per configuration.
   random orientations) against the 20,000-atom protein: energies and
   gradients against the FP64 oracle, central differences on a sample;
 * slab: 300,000 fixed atoms, 150 x 150 x 120 A, PME (1 A grid, beta 0.34,
-  12 A cutoff): a run window in Ewald mode, then trial energies against an
-  FP64 restatement of the same PME discretisation plus FP64 real-space terms.
+  12 A cutoff): a 64-move window in the graph form (equal to the plain
+  form's chain), then trial energies against an FP64 restatement of the same
+  PME discretisation plus FP64 real-space terms.
 
 Bars as in tests/lockstep.py (energies 4.184e-4 kJ/mol for |E| <= 200
 kJ/mol, 1e-5 relative above; decisions >= 99.99 %; poses, inclusion sets,
@@ -177,14 +178,25 @@ def test_adjoint_config_4096_poses():
 def test_slab_config_ewald():
     beta, spacing = 0.34, 1.0
     s = synth.slab(mcsteps=50000)
+    # the plain form, then the graph form (two captured 32-move chunks) from
+    # the same start: the same chain
+    p = systems.copy(s)
+    eng = Engine()
+    eng.add_system(p)
+    eng.set_ewald(0, beta, spacing)
+    eng.upload()
+    eng.run(64, 'plain')
+    eng.download(0)
+    eng.close()
     g = systems.copy(s)
     eng = Engine()
     eng.add_system(g)
     eng.set_ewald(0, beta, spacing)
     eng.upload()
     try:
-        eng.run(60, 'plain')
+        eng.run(64, 'graph')
         eng.download(0)
+        assert np.array_equal(g['atoms'].view(np.uint8), p['atoms'].view(np.uint8))
         st = eng.stats(0)
         L = g['info'][0]['cryst'].astype(np.float64)
         K = [max(4, int(round(float(x) / spacing))) for x in g['info'][0]['cryst']]
@@ -205,7 +217,7 @@ def test_slab_config_ewald():
             n += len(c)
     finally:
         eng.close()
-    print(f'slab (300k atoms, PME 1 A): {int(g["frag"]["totalNum"].sum())} molecules after 60 moves, '
+    print(f'slab (300k atoms, PME 1 A): {int(g["frag"]["totalNum"].sum())} molecules after 64 moves, '
           f'{int(st["accepts"].sum())} accepted; {n} trial energies vs FP64 PME + real space: max |dE| '
           f'{worst:.2e} kJ/mol (|E| <= 200), rel {rel:.1e} above', flush=True)
     assert st['accepts'].sum() > 0

@@ -1,6 +1,6 @@
 """Ewald mode of the engine (real-space erfc pair terms + PME reciprocal
-potential) against FP64 Ewald sums, and the exactness of its incremental
-PME updates.
+potential) against FP64 Ewald sums, the exactness of its incremental PME
+updates, and its batched and graph forms against the plain form.
 
 Bars: trial energies within the PME discretisation error of order-4
 B-splines at beta = 0.34 /A -- 0.15 kJ/mol at 1 A grid spacing, 0.01 kJ/mol
@@ -30,6 +30,10 @@ BETA = 0.34
 def _state():
     s = systems.multi(nsteps=400, n_fixed=1500, box=32.0, seed=77, conf_scale=0.2)
     O.run(s, first=0, nsteps=250)           # populate with some molecules (plain chain)
+    # runs from this state replay the schedule from move 0: another seed, or
+    # an insertion trial would land exactly on the molecule the same
+    # (seed, move, trial) counter inserted above (r = 0: an infinite energy)
+    s['info'][0]['seed'] = np.uint32(int(s['info'][0]['seed']) ^ 0x5bd1e995)
     return s
 
 
@@ -125,9 +129,60 @@ def test_ewald_run_reproducible_and_grid_exact():
     assert np.array_equal(phi0.view(np.uint32), phi_f.view(np.uint32))
 
 
-def test_ewald_requires_plain_mode():
+def _ewald_run(systems_, mode, chunks, spacing=1.0, trace=0):
+    eng = Engine()
+    gs = [systems.copy(x) for x in systems_]
+    for r, g in enumerate(gs):
+        eng.add_system(g)
+        eng.set_ewald(r, BETA, spacing)
+    eng.upload()
+    if trace:
+        for r, g in enumerate(gs):
+            eng.set_trace(r, trace, int(g['frag']['confBias'].max()))
+    for n in chunks:
+        eng.run(n, mode)
+    pts = np.random.default_rng(3).random((64, 3)).astype(np.float32) * gs[0]['info'][0]['cryst']
+    phi = [eng.ewald_potential(r, pts) for r in range(len(gs))]
+    trs = [eng.get_trace(r) for r in range(len(gs))] if trace else None
+    for r in range(len(gs)):
+        eng.download(r)
+    st = [eng.stats(r) for r in range(len(gs))]
+    eng.close()
+    return gs, phi, trs, st
+
+
+def test_ewald_batched_and_graph_forms_equal_plain():
+    """Ewald systems in the batched form (move by move, each move's trials in
+    one launch, PME update between moves) and the graph form (a captured
+    chunk of moves replayed; 100 moves = three 32-move replays + 4 eager
+    moves, then 30 more eager): the same trial energies, accept flags, final
+    states and reciprocal potentials as the plain form, bit for bit, two
+    systems at once."""
+    a = _state()
+    b = systems.multi(nsteps=400, n_fixed=1200, box=30.0, seed=78, conf_scale=0.2)
+    outs = {}
+    for mode, chunks in (('plain', (100, 30)), ('batched', (100, 30)), ('graph', (100, 30))):
+        outs[mode] = _ewald_run([a, b], mode, chunks, trace=130)
+    gp, phip, trp, stp = outs['plain']
+    assert sum(int(x['accepts'].sum()) for x in stp) > 0
+    for mode in ('batched', 'graph'):
+        g, phi, tr, st = outs[mode]
+        for r in range(2):
+            assert np.array_equal(tr[r]['e'].view(np.uint64), trp[r]['e'].view(np.uint64)), (mode, r)
+            assert np.array_equal(tr[r]['gacc'], trp[r]['gacc']), (mode, r)
+            assert np.array_equal(g[r]['atoms'].view(np.uint8), gp[r]['atoms'].view(np.uint8)), (mode, r)
+            assert np.array_equal(phi[r].view(np.uint32), phip[r].view(np.uint32)), (mode, r)
+            assert st[r]['energy'] == stp[r]['energy']
+
+
+def test_ewald_and_plain_systems_do_not_mix_in_batched_form():
     s = _state()
-    eng = _engine(s, 1.0)
-    with pytest.raises(RuntimeError, match='plain mode'):
+    eng = Engine()
+    eng.add_system(systems.copy(s))
+    eng.add_system(systems.copy(s))
+    eng.set_ewald(0, BETA, 1.0)
+    eng.upload()
+    with pytest.raises(RuntimeError, match='all systems with Ewald'):
         eng.run(5, 'batched')
+    eng.run(5, 'plain')
     eng.close()
