@@ -320,8 +320,10 @@ int gcmc_host_within(int n, const float *a, const float *b, const float *cryst, 
 
 /* Host check of the cell-list superset property for n (centre, group) pairs:
  * out[i] = 1 when the group's cell is one of the stencil cells the energy
- * kernel scans around the centre's cell (box cryst from startxyz, target
- * cell edge cell_size).  Every pair that passes gcmc_host_within must give 1. */
+ * kernel scans for the centre: a stencil cell around the centre's cell (box
+ * cryst from startxyz, target cell edge cell_size) whose box is not pruned
+ * by its distance from the centre itself.  Every pair that passes
+ * gcmc_host_within must give 1. */
 int gcmc_host_stencil_covers(int n, const float *centres, const float *groups, const float *cryst,
                              const float *startxyz, float cutoff, float cell_size,
                              unsigned char *out);

@@ -64,7 +64,7 @@ struct AdjPass {
     float gx[NF], gy[NF], gz[NF], sr[NF], se[NF], sq[NF];
 
     template <int NT, bool STAGED>
-    __device__ __attribute__((always_inline)) int run(const Rep &C, const Lds &L, int n, int nf, float &acc,
+    __device__ __attribute__((always_inline)) int run(CRep &C, const Lds &L, int n, int nf, float &acc,
                                                       float &cmp, double &acc64)
     {
         const int At = wg_scan<NT>(L.incn, n, L.misc);
@@ -147,11 +147,11 @@ __global__ __launch_bounds__(NT) void k_adjoint(const Rep *reps, int r, int spi,
 {
     extern __shared__ __align__(16) char smem[];
     const Lds L = carve(smem, lay);
-    const Rep &C = reps[r];
+    CRep &C = rep_of(reps, r);
     const int X = C.X, Y = C.Y;
     AdjPass<NF> P;
     P.A = adj_carve(smem + adj_off, X, Y);
-    const Species &F = C.sp[spi];
+    CSpecies &F = species_of(C, spi);
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     for (int t = tid; t < Y; t += NT) {
         const float ek = fabsf(eps_kcal[t]);

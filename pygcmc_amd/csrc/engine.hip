@@ -209,6 +209,20 @@ void cell_geometry(const float cryst[3], float cell_size, int nc[3], float inv_h
     }
 }
 
+// Trial-specific pruning of the stencil (stencil_cell_near): cell edges, the
+// axes whose stencil offsets are the minimum-image ones (the stencil does not
+// wrap onto itself there) and the squared reach rc + slack.
+void stencil_prune(const float L[3], const int nc[3], float rc, float h[3], int *prune, float *r2)
+{
+    *prune = 0;
+    for (int i = 0; i < 3; ++i) {
+        h[i] = L[i] / (float)nc[i];
+        const int r = (int)std::floor((rc + STENCIL_EPS) / ((double)L[i] / nc[i])) + 1;
+        if (2 * r + 1 < nc[i]) *prune |= 1 << i;
+    }
+    *r2 = (rc + STENCIL_EPS) * (rc + STENCIL_EPS);
+}
+
 // cell stencil: offsets whose cell box lies within rc + eps of the centre cell
 std::vector<int> make_stencil(const float L[3], const int nc[3], float rc, float eps)
 {
@@ -431,6 +445,12 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     }
     std::vector<int> st = make_stencil(R.L, R.nc, R.rc, STENCIL_EPS);
     R.nst = (int)st.size();
+    // Per trial, a stencil cell whose box lies farther than rc + slack from
+    // the trial centre holds no group that passes the group test: skipped.
+    // Along an axis whose stencil offsets are the minimum-image ones (it does
+    // not wrap onto itself) the box distance is measured from the centre's
+    // position in its cell; other axes count no distance.
+    stencil_prune(R.L, R.nc, R.rc, R.cell_h, &R.prune, &R.prune_r2);
 
     DevBuf &B = H.buf;
     HIPX(put((float **)&R.grid, B, H.grid.data(), H.grid.size()));
@@ -1546,12 +1566,20 @@ int gcmc_host_stencil_covers(int n, const float *centres, const float *groups, c
     }
     cell_geometry(L, cell_size, nc, inv_h);
     const std::vector<int> sten = make_stencil(L, nc, cutoff, STENCIL_EPS);
+    float h[3], r2;
+    int prune;
+    stencil_prune(L, nc, cutoff, h, &prune, &r2);
     for (int q = 0; q < n; ++q) {
         const int cc = cell_index(L, st, inv_h, nc, centres[3 * q], centres[3 * q + 1], centres[3 * q + 2]);
         const int cg = cell_index(L, st, inv_h, nc, groups[3 * q], groups[3 * q + 1], groups[3 * q + 2]);
         const int c0 = cc / (nc[1] * nc[2]), c1 = (cc / nc[2]) % nc[1], c2 = cc % nc[2];
+        const int ci[3] = {c0, c1, c2};
+        float u[3];
+        cell_offset(L, st, h, ci, centres[3 * q], centres[3 * q + 1], centres[3 * q + 2], u);
         out[q] = 0;
         for (int code : sten) {
+            const int d[3] = {(code & 255) - 128, ((code >> 8) & 255) - 128, ((code >> 16) & 255) - 128};
+            if (!stencil_cell_near(u, d, h, prune, r2)) continue;     // the kernel skips this cell
             int i0 = c0 + (code & 255) - 128, i1 = c1 + ((code >> 8) & 255) - 128, i2 = c2 + ((code >> 16) & 255) - 128;
             i0 = ((i0 % nc[0]) + nc[0]) % nc[0];
             i1 = ((i1 % nc[1]) + nc[1]) % nc[1];

@@ -40,6 +40,20 @@ __device__ __forceinline__ gptr<T> gbl(T *p)
     return (gptr<T>)p;
 }
 
+// A replica's Rep and species table are never written while kernels run.
+// Read through the constant address space, a field at a wave-uniform address
+// is a scalar load (scalar data cache) rather than a vector load through the
+// texture path that the evaluations' gathers keep busy.
+#if defined(__HIP_DEVICE_COMPILE__)
+using CRep = const __attribute__((address_space(4))) Rep;
+using CSpecies = const __attribute__((address_space(4))) Species;
+#else
+using CRep = const Rep;
+using CSpecies = const Species;
+#endif
+__device__ __forceinline__ CRep &rep_of(const Rep *reps, int r) { return ((CRep *)reps)[r]; }
+__device__ __forceinline__ CSpecies &species_of(CRep &C, int s) { return ((CSpecies *)C.sp)[s]; }
+
 // scratch ints of a workgroup (Lds::misc)
 enum Misc : int {
     M_SCAN = 0,      // [0, NW]   scan partials, total at NW
@@ -298,7 +312,7 @@ __device__ inline float two_diff(float a, float b, float &e)
 // lane <-> environment atom (float4 load, coalesced within a group), loop
 // over the fragment atoms held in LDS; FP32 terms, TwoSum into (acc, cmp).
 template <int NT, bool STAGED, bool EW>
-__device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
+__device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
 {
     const int A = wg_scan<NT>(L.incn, n, L.misc, L.amap, L.map_cap);
     const bool mapped = A <= L.map_cap;
@@ -416,17 +430,17 @@ __device__ inline int pair_pass(const Rep &C, const Lds &L, int n, int nf, float
 template <bool EW>
 struct EnergyPass {
     template <int NT, bool STAGED>
-    __device__ int run(const Rep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
+    __device__ int run(CRep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
     {
         return pair_pass<NT, STAGED, EW>(C, L, n, nf, acc, cmp, acc64);
     }
 };
 
 // species table in LDS (mobile group id -> atom range)
-__device__ inline void load_species_table(const Rep &C, const Lds &L)
+__device__ inline void load_species_table(CRep &C, const Lds &L)
 {
     for (int s = threadIdx.x; s < C.T; s += blockDim.x) {
-        const Species &F = C.sp[s];
+        CSpecies &F = species_of(C, s);
         L.spt[s] = make_int4(F.startRes, F.maxNum, F.atomBase, F.nf);
     }
 }
@@ -434,7 +448,7 @@ __device__ inline void load_species_table(const Rep &C, const Lds &L)
 // pair-table rows of species spi's distinct atom types into LDS (atom f's
 // row at L.frow[f] = F.urow[f] when staged)
 template <int NT>
-__device__ inline void stage_species_rows(const Rep &C, int spi, int nu, const Lds &L)
+__device__ inline void stage_species_rows(CRep &C, int spi, int nu, const Lds &L)
 {
     const auto SP = gbl(C.spair) + (size_t)spi * MAXF * C.Y;
     const int n = nu * C.Y;
@@ -447,7 +461,7 @@ __device__ inline void stage_species_rows(const Rep &C, int spi, int nu, const L
 // distanceP test passes (gcmc_energy.h:152-191).  Needs L.spt loaded.
 // Thread 0 receives the results.
 template <int NT, bool STAGED, class Pass>
-__device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep &C, float cx, float cy, float cz,
+__device__ __attribute__((always_inline)) inline void eval_energy_pass(CRep &C, float cx, float cy, float cz,
                                                                        int nf, int exclude, const Lds &L, Pass &P,
                                                                        double *outE, int *outN, uint32_t *outH,
                                                                        int *outA, unsigned long long *ticks)
@@ -463,24 +477,40 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
     const int c2 = cc % C.nc[2];
     const int c1 = (cc / C.nc[2]) % C.nc[1];
     const int c0 = cc / (C.nc[1] * C.nc[2]);
+    // the centre's position inside its cell
+    float u[3];
+    {
+        const int ci[3] = {c0, c1, c2};
+        cell_offset(C.L, C.start, C.cell_h, ci, cx, cy, cz, u);
+    }
     const int ne = C.nst + 1;
     for (int e = tid; e < ne; e += NT) {
         int cell, fs, fc, mc;
         if (e < C.nst) {
             const int code = stencil[e];
-            // stencil offsets span at most one box per axis: wrap by one add
-            int i0 = c0 + (code & 255) - 128;
-            int i1 = c1 + ((code >> 8) & 255) - 128;
-            int i2 = c2 + ((code >> 16) & 255) - 128;
-            i0 += i0 < 0 ? C.nc[0] : (i0 >= C.nc[0] ? -C.nc[0] : 0);
-            i1 += i1 < 0 ? C.nc[1] : (i1 >= C.nc[1] ? -C.nc[1] : 0);
-            i2 += i2 < 0 ? C.nc[2] : (i2 >= C.nc[2] ? -C.nc[2] : 0);
-            cell = (i0 * C.nc[1] + i1) * C.nc[2] + i2;
-            const int2 ci = cellv[cell];
-            fs = ci.x;
-            fc = cellv[cell + 1].x - fs;
-            mc = ci.y;
-            L.smob[e] = cell * C.cap;
+            const int d[3] = {(code & 255) - 128, ((code >> 8) & 255) - 128, ((code >> 16) & 255) - 128};
+            // cells whose box lies beyond rc + slack from the centre hold no
+            // group centre that passes the test: no loads for them
+            if (!stencil_cell_near(u, d, C.cell_h, C.prune, C.prune_r2)) {
+                fs = 0;
+                fc = 0;
+                mc = 0;
+                L.smob[e] = 0;
+            } else {
+                // stencil offsets span at most one box per axis: wrap by one add
+                int i0 = c0 + d[0];
+                int i1 = c1 + d[1];
+                int i2 = c2 + d[2];
+                i0 += i0 < 0 ? C.nc[0] : (i0 >= C.nc[0] ? -C.nc[0] : 0);
+                i1 += i1 < 0 ? C.nc[1] : (i1 >= C.nc[1] ? -C.nc[1] : 0);
+                i2 += i2 < 0 ? C.nc[2] : (i2 >= C.nc[2] ? -C.nc[2] : 0);
+                cell = (i0 * C.nc[1] + i1) * C.nc[2] + i2;
+                const int2 ci = cellv[cell];
+                fs = ci.x;
+                fc = cellv[cell + 1].x - fs;
+                mc = ci.y;
+                L.smob[e] = cell * C.cap;
+            }
         } else {
             fs = 0;
             fc = 0;
@@ -604,7 +634,7 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(const Rep
 }
 
 template <int NT, bool STAGED, bool EW = false>
-__device__ __attribute__((always_inline)) inline void eval_energy(const Rep &C, float cx, float cy, float cz, int nf,
+__device__ __attribute__((always_inline)) inline void eval_energy(CRep &C, float cx, float cy, float cz, int nf,
                                                                   int exclude, const Lds &L, double *outE, int *outN,
                                                                   uint32_t *outH, int *outA,
                                                                   unsigned long long *ticks = nullptr)
@@ -624,7 +654,7 @@ __device__ inline uint32_t pose_hash(const float4 *p, int nf)
     return h;
 }
 
-__device__ inline bool tracing(const Rep &C, int step, int &row)
+__device__ inline bool tracing(CRep &C, int step, int &row)
 {
     row = step - C.tr.first;
     return C.tr.k != nullptr && row >= 0 && row < C.tr.max_steps;
@@ -668,7 +698,7 @@ __device__ inline void add_item_stats(State &W, const ItemStats &s)
 // of the species, left out alike on insertion and deletion.  Called by every
 // thread; thread 0 gets the sum.
 template <int NT>
-__device__ inline double ewald_recip(const Rep &C, const Species &F, const Lds &L, int nf, int base)
+__device__ inline double ewald_recip(CRep &C, CSpecies &F, const Lds &L, int nf, int base)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double v = 0.0;
@@ -704,13 +734,13 @@ __device__ inline double ewald_recip(const Rep &C, const Species &F, const Lds &
 template <int NT, bool STAGED, bool LOCAL, bool EW>
 // Thread 0 adds the statistics to S, or leaves them in *defer (thread 0's
 // copy) for the caller to add once the item's completion is published.
-__device__ inline void eval_item(const Rep &C, const State &S, int item, const Lds &L, bool stage_rep = true,
+__device__ inline void eval_item(CRep &C, const State &S, int item, const Lds &L, bool stage_rep = true,
                                  bool stage_rows = true, ItemStats *defer = nullptr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long tq0 = wall_clock64();
     const int kind = S.kind, spi = S.sp, k = S.k, step = S.step;
-    const auto &F = gbl(C.sp)[spi];
+    CSpecies &F = species_of(C, spi);
     const int nf = F.nf;
     const auto tc = gbl(C.tc);
     const auto atom = gbl(C.atom);
@@ -855,7 +885,7 @@ __device__ inline void eval_item(const Rep &C, const State &S, int item, const L
 }
 
 // ---- cell-list maintenance (single thread) ----------------------------------
-__device__ inline void cell_insert(const Rep &C, int gid, int spi, float x, float y, float z)
+__device__ inline void cell_insert(CRep &C, int gid, int spi, float x, float y, float z)
 {
     const int c = cell_index(C.L, C.start, C.inv_h, C.nc, x, y, z);
     const int n = C.cell[c].y;
@@ -873,7 +903,7 @@ __device__ inline void cell_insert(const Rep &C, int gid, int spi, float x, floa
     C.mob_loc[gid] = pos;
 }
 
-__device__ inline void cell_remove(const Rep &C, int gid)
+__device__ inline void cell_remove(CRep &C, int gid)
 {
     const int pos = C.mob_loc[gid];
     int c, base;
@@ -894,7 +924,7 @@ __device__ inline void cell_remove(const Rep &C, int gid)
 }
 
 // group `from` now lives in slot `to` (its centre and species are unchanged)
-__device__ inline void cell_rename(const Rep &C, int from, int to)
+__device__ inline void cell_rename(CRep &C, int from, int to)
 {
     const int pos = C.mob_loc[from];
     C.mob_slot[pos] = to;
@@ -917,7 +947,7 @@ __device__ inline void cell_rename(const Rep &C, int from, int to)
 // are wave-parallel: lane l sums a contiguous chunk of the members in order,
 // the chunk sums are scanned across lanes, and the lane whose chunk holds
 // the crossing walks it -- one fixed FP64 summation order.
-__device__ inline uint32_t adj_word(const Rep &C, const Lds &L, int nw, int t, int w)
+__device__ inline uint32_t adj_word(CRep &C, const Lds &L, int nw, int t, int w)
 {
     return L.adjl ? L.adjl[t * nw + w] : C.adj[t * ADJW + w];
 }
@@ -930,7 +960,7 @@ struct MoveCtx {
     double B, energy;
 };
 
-__device__ inline MoveCtx move_ctx(const Rep &C, const State &S)
+__device__ inline MoveCtx move_ctx(CRep &C, const State &S)
 {
     MoveCtx m;
     m.kind = S.kind;
@@ -939,7 +969,7 @@ __device__ inline MoveCtx move_ctx(const Rep &C, const State &S)
     m.step = S.step;
     m.N = S.totalNum[m.spi];
     m.energy = S.energy;
-    const auto &F = gbl(C.sp)[m.spi];
+    CSpecies &F = species_of(C, m.spi);
     m.nf = F.nf;
     m.maxN = F.maxNum;
     m.startRes = F.startRes;
@@ -952,11 +982,11 @@ __device__ inline MoveCtx move_ctx(const Rep &C, const State &S)
 // phase's selection wave is busy meanwhile).  The species count is the one
 // before the current move: prep takes the current move's result for its own
 // species.
-__device__ inline void load_next_move(const Rep &C, const State &S, const Lds &L, const MoveCtx &m)
+__device__ inline void load_next_move(CRep &C, const State &S, const Lds &L, const MoveCtx &m)
 {
     const int end = S.end, step = m.step + 1;
     const int mv = step < end ? C.moves[step] : 0;
-    const auto &F = gbl(C.sp)[mv >> 2];
+    CSpecies &F = species_of(C, mv >> 2);
     L.misc[M_NX + 0] = end;
     L.misc[M_NX + 1] = mv;
     L.misc[M_NX + 2] = S.totalNum[mv >> 2];
@@ -965,7 +995,7 @@ __device__ inline void load_next_move(const Rep &C, const State &S, const Lds &L
     L.misc[M_NX + 5] = F.startRes;
 }
 
-__device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc)
+__device__ inline void greedy_select(CRep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc)
 {
     const int lane = threadIdx.x & 63;
     const int kind = mc.kind, k = mc.k;
@@ -1099,7 +1129,7 @@ __device__ inline void greedy_select(const Rep &C, State &S, const Lds &L, int r
 // cutoff) and slot, and group t's two variates.  Group members are compacted
 // into lanes 0..m-1 in trial order (ds_permute), so the weights, their sums
 // and the decisions are those of greedy_select, bit for bit.  Wave 0 only.
-__device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc,
+__device__ inline void greedy_select64(CRep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc,
                                        double val, double ea, double rsel, double racc, unsigned long long adj,
                                        int tslot)
 {
@@ -1191,11 +1221,11 @@ __device__ inline void greedy_select64(const Rep &C, State &S, const Lds &L, int
 // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses, then the group
 // centres and cell-list entries in acceptance order (thread 0).
 template <int NT, bool EW>
-__device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc)
+__device__ inline void commit_accepted(CRep &C, State &S, const Lds &L, int row, bool trace, const MoveCtx &mc)
 {
     const int tid = threadIdx.x;
     const int kind = mc.kind, spi = mc.spi;
-    const Species &F = C.sp[spi];
+    CSpecies &F = species_of(C, spi);
     const int nf = mc.nf;
     const int na = L.misc[M_ACC];
     // GupdateAdd / GupdateTrn / GupdateRot: copy committed poses
@@ -1242,11 +1272,11 @@ __device__ inline void commit_accepted(const Rep &C, State &S, const Lds &L, int
 
 // ---- accept phase: decisions and commit ------------------------------------------
 template <int NT, bool EW>
-__device__ inline void accept_move(const Rep &C, State &S, const Lds &L, const MoveCtx &mc, bool prefetch = false)
+__device__ inline void accept_move(CRep &C, State &S, const Lds &L, const MoveCtx &mc, bool prefetch = false)
 {
     const int tid = threadIdx.x;
     const int kind = mc.kind, spi = mc.spi, k = mc.k, step = mc.step;
-    const Species &F = C.sp[spi];
+    CSpecies &F = species_of(C, spi);
     const int nf = mc.nf;
     int row;
     const bool trace = tracing(C, step, row);
@@ -1383,7 +1413,7 @@ __device__ inline void accept_move(const Rep &C, State &S, const Lds &L, const M
 }
 
 template <int NT, bool EW>
-__device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
+__device__ inline void accept_step(CRep &C, State &S, const Lds &L)
 {
     const MoveCtx mc = move_ctx(C, S);
     accept_move<NT, EW>(C, S, L, mc);
@@ -1394,7 +1424,7 @@ __device__ inline void accept_step(const Rep &C, State &S, const Lds &L)
 // species, the count that move's accept phase left in L.misc[M_N]), or null
 // use_pre: the move after prev (species prev_spi) was loaded into L.misc[M_NX..]
 template <int NT>
-__device__ inline int prep_step(const Rep &C, State &S, const Lds &L, bool use_pre, int prev_step, int prev_spi)
+__device__ inline int prep_step(CRep &C, State &S, const Lds &L, bool use_pre, int prev_step, int prev_spi)
 {
     const int tid = threadIdx.x;
     int step, mv, N, conf, maxNum, startRes;
@@ -1408,7 +1438,7 @@ __device__ inline int prep_step(const Rep &C, State &S, const Lds &L, bool use_p
     } else {
         step = S.step;
         mv = C.moves[step];
-        const Species &F0 = C.sp[mv >> 2];
+        CSpecies &F0 = species_of(C, mv >> 2);
         N = S.totalNum[mv >> 2];
         conf = F0.conf;
         maxNum = F0.maxNum;
@@ -1474,7 +1504,7 @@ __device__ inline int prep_step(const Rep &C, State &S, const Lds &L, bool use_p
 }
 
 template <int NT>
-__device__ inline int prep_step(const Rep &C, State &S, const Lds &L)
+__device__ inline int prep_step(CRep &C, State &S, const Lds &L)
 {
     return prep_step<NT>(C, S, L, false, 0, 0);
 }
@@ -1491,7 +1521,7 @@ __device__ inline void xcd_acquire() { asm volatile("buffer_inv sc1" ::: "memory
 // claim-word priority of a replica's published move: lower is claimed first,
 // the replica with the most estimated work left first -- the run lasts as long
 // as its slowest chain
-__device__ inline int claim_prio(const Rep &C, int step, int end)
+__device__ inline int claim_prio(CRep &C, int step, int end)
 {
     const uint32_t left = C.wcum[end] - C.wcum[step];
     return 0xFFFFFF - (int)(left < 0xFFFFFFu ? left : 0xFFFFFFu);
@@ -1500,7 +1530,7 @@ __device__ inline int claim_prio(const Rep &C, int step, int end)
 // advance past empty moves and publish the next move with work (batched form)
 // use_pre: the move after prev_step (species prev_spi) is in L.misc[M_NX..]
 template <int NT>
-__device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L, bool use_pre,
+__device__ inline void advance_and_publish(CRep &C, State &S, Sched *sch, const Lds &L, bool use_pre,
                                            int prev_step, int prev_spi)
 {
     while (true) {
@@ -1541,7 +1571,7 @@ __device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, c
 }
 
 template <int NT>
-__device__ inline void advance_and_publish(const Rep &C, State &S, Sched *sch, const Lds &L)
+__device__ inline void advance_and_publish(CRep &C, State &S, Sched *sch, const Lds &L)
 {
     advance_and_publish<NT>(C, S, sch, L, false, 0, 0);
 }
@@ -1572,8 +1602,8 @@ __global__ __launch_bounds__(NT) void k_start(const Rep *reps, State *st, Sched 
     extern __shared__ __align__(16) char smem[];
     const Lds L = carve(smem, lay);
     const int r = r0 + blockIdx.x;
-    if (batched) advance_and_publish<NT>(reps[r], st[r], sch, L);
-    else if (st[r].step < st[r].end) prep_step<NT>(reps[r], st[r], L);
+    if (batched) advance_and_publish<NT>(rep_of(reps, r), st[r], sch, L);
+    else if (st[r].step < st[r].end) prep_step<NT>(rep_of(reps, r), st[r], L);
 }
 
 template <int NT, bool STAGED, bool EW>
@@ -1583,7 +1613,7 @@ __global__ __launch_bounds__(NT) void k_eval_plain(const Rep *reps, const State 
     const State &S = st[r];
     if (S.step >= S.end || (int)blockIdx.x >= S.nitems) return;
     const Lds L = carve(smem, lay);
-    eval_item<NT, STAGED, false, EW>(reps[r], S, blockIdx.x, L);
+    eval_item<NT, STAGED, false, EW>(rep_of(reps, r), S, blockIdx.x, L);
 }
 
 template <int NT, bool STAGED, bool EW>
@@ -1593,8 +1623,8 @@ __global__ __launch_bounds__(NT) void k_accept_plain(const Rep *reps, State *st,
     State &S = st[r];
     if (S.step >= S.end) return;
     const Lds L = carve(smem, lay);
-    accept_step<NT, EW>(reps[r], S, L);
-    if (S.step < S.end) prep_step<NT>(reps[r], S, L);
+    accept_step<NT, EW>(rep_of(reps, r), S, L);
+    if (S.step < S.end) prep_step<NT>(rep_of(reps, r), S, L);
 }
 
 // Which replicas a workgroup serves: the XCD it runs on (XCC_ID hardware
@@ -1708,8 +1738,9 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             }
         }
         __syncthreads();
-        const int found = L.misc[M_FOUND];
-        const int item = L.misc[M_ITEM];
+        // uniform across the workgroup: the replica's fields become scalar loads
+        const int found = __builtin_amdgcn_readfirstlane(L.misc[M_FOUND]);
+        const int item = __builtin_amdgcn_readfirstlane(L.misc[M_ITEM]);
         __syncthreads();
         if (found == -2) break;
         if (found == -1) {
@@ -1720,7 +1751,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             continue;
         }
         xcd_acquire();
-        const Rep &C = reps[found];
+        CRep &C = rep_of(reps, found);
         State &S = st[found];
         const unsigned long long te0 = wall_clock64();
         const int rows = found * MAXT + S.sp;
@@ -1790,8 +1821,8 @@ __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int s
 {
     extern __shared__ __align__(16) char smem[];
     const Lds L = carve(smem, lay);
-    const Rep &C = reps[r];
-    const Species &F = C.sp[spi];
+    CRep &C = rep_of(reps, r);
+    CSpecies &F = species_of(C, spi);
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x;
     if (STAGED) stage_species_rows<NT>(C, spi, F.nu, L);
     load_species_table(C, L);

@@ -79,7 +79,10 @@ struct Rep {
     int G, T, X, Y;
     int nfix, nres, natom, maxk;
     int cap, ovf_base, nst, nmoves;
-    int coul64, pad1;        // Coulomb pair terms in FP64
+    int coul64, prune;       // Coulomb pair terms in FP64; bit i: stencil cells may be
+                             // pruned along axis i by their distance from the trial centre
+    float cell_h[3];         // cell edges L / nc
+    float prune_r2;          // (rc + stencil slack)^2
     uint32_t seed;
     int pad0;
     const float *grid;
@@ -186,6 +189,34 @@ __host__ __device__ inline int cell_index(const float L[3], const float start[3]
         c[i] = ci < 0 ? 0 : (ci >= nc[i] ? nc[i] - 1 : ci);
     }
     return (c[0] * nc[1] + c[1]) * nc[2] + c[2];
+}
+
+// position of (x, y, z) inside its cell c[3] (cell_index's wrap), per axis
+__host__ __device__ inline void cell_offset(const float L[3], const float start[3], const float h[3],
+                                            const int c[3], float x, float y, float z, float u[3])
+{
+    const float p[3] = {x, y, z};
+    for (int i = 0; i < 3; ++i) {
+        float w = p[i] - start[i];
+        w = w - floorf(w / L[i]) * L[i];
+        u[i] = w - (float)c[i] * h[i];
+    }
+}
+
+// Whether the cell at stencil offset d[3] from a centre at position u[3]
+// inside its cell may hold a group centre within the cutoff: false when the
+// cell's box lies farther than sqrt(r2) (rc + slack) from the centre along
+// the axes whose bit is set in `prune`, true otherwise.
+__host__ __device__ inline bool stencil_cell_near(const float u[3], const int d[3], const float h[3], int prune,
+                                                  float r2)
+{
+    float g2 = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+        if (!((prune >> i) & 1)) continue;
+        const float g = d[i] > 0 ? (float)d[i] * h[i] - u[i] : (d[i] < 0 ? u[i] - (float)(d[i] + 1) * h[i] : 0.0f);
+        g2 = fmaf(g, g, g2);
+    }
+    return g2 <= r2;
 }
 
 }  // namespace gcmc

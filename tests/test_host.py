@@ -124,3 +124,37 @@ def test_fragment_templates_have_config_atom_counts():
     for name, b, net, _ in synth.FRAGMENTS:
         q = synth.fragment_charges(rng, len(b()[1]), net)
         assert abs(q.sum() - net) < 1e-12
+
+
+@pytest.mark.parametrize('cell', [5.0, 6.0, 7.5])
+def test_pruned_stencil_covers_cutoff_pairs_at_cell_faces(cell):
+    """Centres on and next to cell faces, partners at the cutoff along axis,
+    face-diagonal and body-diagonal directions: the stencil cells kept for
+    the centre (pruned by their box's distance from it) hold every partner
+    that passes the group test."""
+    rng = np.random.default_rng(31)
+    L = np.array([54.31, 80.0, 124.07])
+    start = np.array([-1.25, 0.5, 3.0])
+    rc = 15.0
+    nc = np.maximum(1, np.floor(L / cell)).astype(int)
+    h = L / nc
+    fr = np.array([0.0, 1e-5, 0.25, 0.5, 1.0 - 1e-5])
+    dirs = [np.array(v, np.float64) for v in
+            [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, 1, 0),
+             (1, 0, -1), (0, -1, -1), (1, 1, 1), (-1, -1, 1), (1, -1, -1), (-1, 1, -1)]]
+    a, b = [], []
+    for _ in range(60):
+        c = rng.integers(0, nc)
+        for f in rng.choice(fr, (6, 3)):
+            p = start + (c + f) * h
+            for d in dirs:
+                d = d / np.linalg.norm(d)
+                for s in (1 - 1e-6, 1.0, 1 + 1e-6):
+                    a.append(p)
+                    b.append(p + d * rc * s)
+    a = np.array(a, np.float32)
+    b = np.array(b, np.float32)
+    inc = engine.host_within(a, b, L, rc)
+    cov = engine.host_stencil_covers(a, b, L, start, rc, cell)
+    assert inc.sum() > 1000
+    assert cov[inc].all()
