@@ -97,6 +97,10 @@ typedef struct gcmc_tuning {
                             first; N > 0 = the same, except that a move of at least N unclaimed
                             items of the system whose tables a workgroup holds in LDS comes
                             before other such moves.  Changes the order of work, never results */
+    int xcd_round_robin; /* batched form: 1 = systems to XCDs round-robin by index; 0 (default) =
+                            balanced, heaviest first to the least loaded XCD, by estimated work
+                            before the first run and by the work measured in the last run after
+                            it.  Changes which workgroups serve a system, never results */
 } gcmc_tuning;
 
 #define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */

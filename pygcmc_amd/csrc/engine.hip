@@ -115,6 +115,10 @@ struct HostRep {
     int maxk = 1;
     int host_step = 0;
     size_t nmslot = 0;         // entries of mob_slot / mob_rec
+    double est_cost = 0.0;     // estimated cost of the whole schedule (cost model of the claim priority)
+    unsigned long long n_eval_now = 0;  // the State's evaluation counter as the host last saw it
+    unsigned long long snap_n_eval = 0;
+    double last_work = 0.0;    // measured work of the last batched run: evaluations + moves x MOVE_WORK
     // device snapshot of the state the chain changes (gcmc_engine_snapshot)
     DevBuf snap;
     float4 *s_res_pos = nullptr, *s_atom = nullptr, *s_mob_rec = nullptr;
@@ -166,6 +170,7 @@ struct gcmc_engine {
     XcdMap xm{};
     int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]
     State *d_state_snap = nullptr;  // State of every replica at gcmc_engine_snapshot
+    std::vector<int> cls_host;      // the replica -> XCD class map on the device (cls_start, cls_reps)
     double *d_adj_work = nullptr;  // column-sum partials of the adjoint backward
     size_t adj_work_n = 0;
 };
@@ -724,8 +729,48 @@ struct RunPoses {
     }
 };
 
-// Which XCC ids the device's workgroups run on, and a round-robin assignment
-// of replicas to them (each replica is then served by one XCD only).
+// Replicas to XCD classes (each replica is served by the workgroups of one
+// XCD).  Round-robin when the tuning asks for it; otherwise the heaviest
+// replica first to the class with the least work so far (longest processing
+// time first), work w[r] being the cost model's estimate of the schedule
+// before the first run and the measured work of the last run after it: the
+// run ends when its slowest XCD does, and the XCDs' loads then differ least.
+// Changes which workgroups evaluate a replica's items, never the results.
+int assign_classes(gcmc_engine *E, const std::vector<double> &w)
+{
+    XcdMap &m = E->xm;
+    const int R = (int)E->reps.size();
+    std::vector<std::vector<int>> members(m.nclass);
+    if (E->tune.xcd_round_robin) {
+        for (int r = 0; r < R; ++r) members[r % m.nclass].push_back(r);
+    } else {
+        std::vector<int> order(R);
+        for (int r = 0; r < R; ++r) order[r] = r;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+        std::vector<double> load(m.nclass, 0.0);
+        for (int r : order) {
+            int c = 0;
+            for (int q = 1; q < m.nclass; ++q)
+                if (load[q] < load[c]) c = q;
+            load[c] += w[r];
+            members[c].push_back(r);
+        }
+        for (auto &v : members) std::sort(v.begin(), v.end());
+    }
+    std::vector<int> cls(m.nclass + 1 + R, 0);
+    int o = 0;
+    for (int c = 0; c < m.nclass; ++c) {
+        cls[c] = o;
+        for (int r : members[c]) cls[m.nclass + 1 + o++] = r;
+    }
+    cls[m.nclass] = o;
+    HIPX(hipMemcpy(E->d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice));
+    E->cls_host = cls;
+    return 0;
+}
+
+// Which XCC ids the device's workgroups run on, and the assignment of
+// replicas to them (assign_classes).
 int build_xcd_map(gcmc_engine *E, int ncu)
 {
     const int nprobe = std::max(64, 8 * ncu);
@@ -745,20 +790,12 @@ int build_xcd_map(gcmc_engine *E, int ncu)
         if (id >= 0 && id < 16 && m.xcc_class[id] < 0) m.xcc_class[id] = m.nclass++;
     if (m.nclass == 0) return fail(GCMC_E_DEVICE, "XCC probe returned no ids");
     const int R = (int)E->reps.size();
-    std::vector<int> cls(m.nclass + 1 + R, 0);
-    std::vector<std::vector<int>> members(m.nclass);
-    for (int r = 0; r < R; ++r) members[r % m.nclass].push_back(r);
-    int o = 0;
-    for (int c = 0; c < m.nclass; ++c) {
-        cls[c] = o;
-        for (int r : members[c]) cls[m.nclass + 1 + o++] = r;
-    }
-    cls[m.nclass] = o;
-    HIPX(hipMalloc(&E->d_cls, cls.size() * sizeof(int)));
-    HIPX(hipMemcpy(E->d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPX(hipMalloc(&E->d_cls, (m.nclass + 1 + R) * sizeof(int)));
     m.cls_start = E->d_cls;
     m.cls_reps = E->d_cls + m.nclass + 1;
-    return 0;
+    std::vector<double> w(R);
+    for (int r = 0; r < R; ++r) w[r] = E->reps[r]->est_cost;
+    return assign_classes(E, w);
 }
 
 int read_states(gcmc_engine *E, std::vector<State> &st)
@@ -916,6 +953,7 @@ int gcmc_engine_upload(gcmc_engine *E)
         while ((top >> shift) >= (1ull << 24)) ++shift;
         for (size_t r = 0; r < E->reps.size(); ++r) {
             HostRep &H = *E->reps[r];
+            H.est_cost = (double)cum[r].back();
             std::vector<uint32_t> w(cum[r].size());
             for (size_t i = 0; i < w.size(); ++i) w[i] = (uint32_t)(cum[r][i] >> shift);
             HIPX(put((uint32_t **)&H.rep.wcum, H.buf, w.data(), w.size()));
@@ -964,6 +1002,21 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
                                 "or none; run mixed engines in plain mode");
     if (mode != GCMC_MODE_PLAIN && mode != GCMC_MODE_BATCHED && mode != GCMC_MODE_GRAPH)
         return fail(GCMC_E_ARG, "unknown run mode");
+    if (mode != GCMC_MODE_PLAIN && n_ew == 0 && !E->tune.xcd_round_robin) {
+        // balance the XCDs by the work each replica did in the last run
+        std::vector<double> w(E->reps.size());
+        bool measured = false;
+        for (size_t r = 0; r < w.size(); ++r) {
+            w[r] = E->reps[r]->last_work;
+            measured |= w[r] > 0.0;
+        }
+        if (measured) {
+            int rc = assign_classes(E, w);
+            if (rc) return rc;
+        }
+    }
+    std::vector<int> step0(E->reps.size());
+    for (size_t r = 0; r < step0.size(); ++r) step0[r] = E->reps[r]->host_step;
     HIPX(hipEventRecord(E->ev0, E->stream));
     if (mode != GCMC_MODE_PLAIN && n_ew > 0) {
         // moves this call runs: the most any system has left
@@ -1022,7 +1075,13 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
     HIPX(hipMemcpy(&sch, E->d_sched, sizeof(sch), hipMemcpyDeviceToHost));
     if (sch.abort) return fail(GCMC_E_DEVICE, "persistent kernel stopped at its spin limit");
     for (size_t r = 0; r < st.size(); ++r) {
-        E->reps[r]->host_step = st[r].step;
+        HostRep &H = *E->reps[r];
+        H.host_step = st[r].step;
+        // work of this run: evaluations, plus one evaluation's worth per move
+        // for its accept and prep (counts, not times: measured times would
+        // follow the contention of the assignment they were measured under)
+        H.last_work = (double)(st[r].n_eval - H.n_eval_now) + (double)(st[r].step - step0[r]);
+        H.n_eval_now = st[r].n_eval;
         if (st[r].step != st[r].end)
             return fail(GCMC_E_DEVICE, "replica " + std::to_string(r) + " stopped at step " +
                                            std::to_string(st[r].step) + " before its end " +
@@ -1186,6 +1245,12 @@ int gcmc_engine_snapshot(gcmc_engine *E)
     }
     HIPX(d2d(E->d_state_snap, E->d_state, (size_t)R, E->stream));
     HIPX(hipStreamSynchronize(E->stream));
+    std::vector<State> st;
+    int rc = read_states(E, st);
+    if (rc) return rc;
+    for (int r = 0; r < R; ++r) {
+        E->reps[r]->n_eval_now = E->reps[r]->snap_n_eval = st[r].n_eval;
+    }
     return 0;
 }
 
@@ -1204,6 +1269,7 @@ int gcmc_engine_restore(gcmc_engine *E)
         HIPX(d2d(P.mob_rec, H.s_mob_rec, H.nmslot, E->stream));
         HIPX(d2d(P.mob_loc, H.s_mob_loc, (size_t)P.nres, E->stream));
         H.host_step = H.snap_step;
+        H.n_eval_now = H.snap_n_eval;
     }
     HIPX(d2d(E->d_state, E->d_state_snap, E->reps.size(), E->stream));
     return 0;

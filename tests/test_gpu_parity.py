@@ -277,3 +277,34 @@ def test_snapshot_restore_reruns_identical_chain():
             assert np.array_equal(atoms, atoms0) and np.array_equal(n, n0)
     # the chain moved: something was accepted
     assert runs[0][0][1].sum() > 0
+
+
+def test_xcd_assignment_never_changes_chains():
+    """Systems assigned to XCDs round-robin, by estimated work, or by the work
+    measured in the previous run (the second run of a snapshot): the same
+    final states."""
+    base = [systems.multi(nsteps=200, n_fixed=n, box=b, seed=sd, conf_scale=0.2)
+            for n, b, sd in ((900, 28.0, 41), (2400, 44.0, 42), (1500, 36.0, 43), (1200, 33.0, 44),
+                             (2000, 42.0, 45), (1000, 30.0, 46), (1800, 40.0, 47), (1300, 34.0, 48),
+                             (2200, 43.0, 49), (1100, 31.0, 50))]
+    finals = []
+    for rr in (1, 0):
+        eng = Engine(xcd_round_robin=rr)
+        gs = [systems.copy(b) for b in base]
+        for g in gs:
+            eng.add_system(g)
+        eng.upload()
+        eng.snapshot()
+        for _ in range(2):          # the second run is balanced by the first's measured work
+            eng.restore()
+            eng.run(200, 'batched')
+            out = []
+            for r in range(len(gs)):
+                g = systems.copy(base[r])
+                eng.download(r, g)
+                out.append(g['atoms'].view(np.uint8).copy())
+            finals.append(out)
+        eng.close()
+    for out in finals[1:]:
+        for a, b in zip(out, finals[0]):
+            assert np.array_equal(a, b)
