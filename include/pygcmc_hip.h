@@ -93,10 +93,12 @@ typedef struct gcmc_tuning {
     int wg_size;         /* threads per workgroup: 256, 512 or 1024; 0 = 256 */
     int fp32_coulomb;    /* 1 = Coulomb pair terms in FP32 like the LJ terms; 0 (default) =
                             Coulomb in FP64 from the exactly formed FP32 displacement */
-    int claim_affinity;  /* batched form: 0 = claim the move with the fewest unclaimed items
-                            first; N > 0 = the same, except that a move of at least N unclaimed
-                            items of the system whose tables a workgroup holds in LDS comes
-                            before other such moves.  Changes the order of work, never results */
+    int claim_affinity;  /* batched form: 0 = claim moves with at most 4, then at most 16
+                            unclaimed items first, each class (and the larger moves) by the
+                            system's estimated work left; N > 0 = the same, except that a move
+                            of at least N unclaimed items of the system whose tables a workgroup
+                            holds in LDS comes before other such moves.  Changes the order of
+                            work, never results */
     int xcd_round_robin; /* batched form: 1 = systems to XCDs round-robin by index; 0 (default) =
                             balanced, heaviest first to the least loaded XCD, by estimated work
                             before the first run and by the work measured in the last run after
