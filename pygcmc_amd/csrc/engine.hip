@@ -109,6 +109,7 @@ struct HostRep {
     std::vector<Atom> atoms;
     std::vector<float> grid, ff;
     std::vector<int> moves;
+    std::vector<float4> pair;  // the pair table as uploaded (Rep::tab: replicas with equal tables)
     Rep rep{};
     DevBuf buf;
     DevBuf trbuf;
@@ -387,21 +388,68 @@ int upload_rep(gcmc_engine *E, HostRep &H)
         double nbar = (double)I.cryst[0] * (double)I.cryst[1] * (double)I.cryst[2] *
                       (double)F.conc * 0.0006023;
         S.B = BETA * (double)F.muex + std::log(nbar);
-        int utype[MAXF];
         for (int a = 0; a < F.num_atoms; ++a) {
             const Atom &A = F.atoms[a];
             S.tmpl[a] = make_float4(A.position[0], A.position[1], A.position[2], A.charge);
             S.kq[a] = (float)(KCOUL * (double)A.charge);
             S.trow[a] = A.type * R.Y;
             S.ttype[a] = A.type;
-            // distinct types in order of first appearance: utype[u] is type u's id
-            int u = 0;
-            while (u < S.nu && utype[u] != A.type) ++u;
-            if (u == S.nu) utype[S.nu++] = A.type;
-            S.urow[a] = u * R.Y;
             if (A.type < 0 || A.type >= R.X)
                 return fail(GCMC_E_ARG, "fragment atom type outside the pair table rows");
         }
+    }
+    // Rep::nowrap.  An environment atom of an included group lies within
+    // rc + r_env of the trial centre and a fragment atom within r_frag of it
+    // (minimum image); when rc + r_env + r_frag < L/2 on every axis, the
+    // atom's one image shift around the trial centre is also the pair's.
+    // A fixed group's extent is its atoms' largest distance from its centre.
+    // A molecule keeps its shape under translation and rotation (about its
+    // centroid, centre unchanged), so its atoms stay within rho + D of its
+    // centre: rho its atoms' largest distance from their centroid, D the
+    // centroid's distance from the centre -- of the template for inserted
+    // molecules, of the molecules present at upload.
+    {
+        auto mi = [&](double d, int i) { return d - (double)R.L[i] * std::nearbyint(d / (double)R.L[i]); };
+        auto extent = [&](const double *c, int n, auto pos, bool about_centroid) {
+            double cen[3] = {0.0, 0.0, 0.0}, rho = 0.0, D = 0.0;
+            for (int a = 0; a < n; ++a)
+                for (int i = 0; i < 3; ++i) cen[i] += mi(pos(a, i) - c[i], i) / n;
+            for (int a = 0; a < n; ++a) {
+                double d2 = 0.0;
+                for (int i = 0; i < 3; ++i) {
+                    const double d = mi(pos(a, i) - c[i], i) - (about_centroid ? cen[i] : 0.0);
+                    d2 += d * d;
+                }
+                rho = std::max(rho, std::sqrt(d2));
+            }
+            if (about_centroid) D = std::sqrt(cen[0] * cen[0] + cen[1] * cen[1] + cen[2] * cen[2]);
+            return rho + D;
+        };
+        auto atoms_of = [&](int g) {
+            const residue &r = H.res[g];
+            return [&H, &r](int a, int i) { return (double)H.atoms[r.atomStart + a].position[i]; };
+        };
+        double r_env = 0.0, r_frag = 0.0;
+        for (int g = 0; g < R.nfix; ++g) {
+            const residue &r = H.res[g];
+            const double c[3] = {r.position[0], r.position[1], r.position[2]};
+            r_env = std::max(r_env, extent(c, r.atomNum, atoms_of(g), false));
+        }
+        for (int s = 0; s < T; ++s) {
+            const AtomArray &F = H.frag[s];
+            const double zero[3] = {0.0, 0.0, 0.0};
+            r_frag = std::max(r_frag, extent(zero, F.num_atoms,
+                                             [&F](int a, int i) { return (double)F.atoms[a].position[i]; }, true));
+            for (int j = 0; j < F.totalNum; ++j) {
+                const residue &r = H.res[F.startRes + j];
+                const double c[3] = {r.position[0], r.position[1], r.position[2]};
+                r_frag = std::max(r_frag, extent(c, r.atomNum, atoms_of(F.startRes + j), true));
+            }
+        }
+        r_env = std::max(r_env, r_frag);
+        const double half = 0.5 * std::min({(double)R.L[0], (double)R.L[1], (double)R.L[2]});
+        // 0.1 A covers FP32 rounding of positions and the drift of repeated rotations
+        R.nowrap = (double)R.rc + r_env + r_frag + 0.1 < half ? 1 : 0;
     }
     // fixed groups: CSR by cell
     std::vector<int> fcell(R.nfix);
@@ -466,16 +514,21 @@ int upload_rep(gcmc_engine *E, HostRep &H)
     HIPX(put(&R.atype, B, atype.data(), atype.size()));
     HIPX(put((float4 **)&R.pair, B, pair.data(), pair.size()));
     HIPX(put((Species **)&R.sp, B, sp.data(), sp.size()));
-    {
-        // each species' rows of the pair table, one per distinct atom type in
-        // order of first appearance, contiguous (staging them in LDS then
-        // needs only the species index; atoms of one type share a row)
-        std::vector<float4> spair((size_t)T * MAXF * R.Y, make_float4(0.f, 0.f, 0.f, 0.f));
-        for (int s = 0; s < T; ++s)
-            for (int a = 0; a < sp[s].nf; ++a)
-                for (int y = 0; y < R.Y; ++y)
-                    spair[((size_t)s * MAXF * R.Y + sp[s].urow[a]) + y] = pair[(size_t)sp[s].trow[a] + y];
-        HIPX(put((float4 **)&R.spair, B, spair.data(), spair.size()));
+    // replicas with equal pair tables share one copy held in LDS (Rep::tab):
+    // the index of the first replica (in upload order) with this table
+    H.pair = pair;
+    R.tab = -1;
+    for (size_t q = 0; q < E->reps.size(); ++q) {
+        const HostRep &P = *E->reps[q];
+        if (&P == &H) {
+            R.tab = (int)q;
+            break;
+        }
+        if (P.rep.X == R.X && P.rep.Y == R.Y && P.pair.size() == pair.size() &&
+            std::memcmp(P.pair.data(), pair.data(), pair.size() * sizeof(float4)) == 0) {
+            R.tab = P.rep.tab;
+            break;
+        }
     }
     {
         std::vector<int2> cellv(R.ncell + 1);
@@ -535,22 +588,13 @@ int upload_rep(gcmc_engine *E, HostRep &H)
 
 int compute_layout(gcmc_engine *E)
 {
-    // staged pair table: the rows of one species' distinct atom types (nu x Y entries)
-    int nst = 1, fy = 1;
+    // the pair table held in LDS: X x Y entries of the largest table
+    int nst = 1, xy = 1;
     for (HostRep *H : E->reps) {
         nst = std::max(nst, H->rep.nst);
-        for (const AtomArray &F : H->frag) {
-            // distinct atom types of the species: its staged rows
-            int nu = 0, seen[MAXF];
-            for (int a = 0; a < F.num_atoms; ++a) {
-                int u = 0;
-                while (u < nu && seen[u] != F.atoms[a].type) ++u;
-                if (u == nu) seen[nu++] = F.atoms[a].type;
-            }
-            fy = std::max(fy, nu * H->rep.Y);
-        }
+        xy = std::max(xy, H->rep.X * H->rep.Y);
     }
-    const int pair_bytes = 16 * fy;
+    const int pair_bytes = 16 * xy;
     E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
     auto up = [](int x) { return (x + 15) & ~15; };
     LdsLayout &l = E->lay;
@@ -567,6 +611,8 @@ int compute_layout(gcmc_engine *E)
     l.amap = o; o = up(o + 2 * l.map_cap);
     l.frag = o; o = up(o + 16 * MAXF);
     l.frow = o; o = up(o + 4 * MAXF);
+    o = (o + 31) & ~31;
+    l.fragd = o; o = up(o + 32 * MAXF);
     if (E->staged) {
         l.pair = o;
         o = up(o + pair_bytes);

@@ -73,6 +73,7 @@ struct Lds {
     int *smob, *sfst, *sfix, *soff, *incs, *incn;
     float4 *frag;
     int *frow;
+    double4 *fragd;   // fragment pose minus the trial centre (x, y, z), K q (w), FP64
     float4 *pair;
     int4 *spt;
     int *misc;
@@ -99,6 +100,7 @@ __device__ inline Lds carve(char *b, const LdsLayout &l)
     s.incn = (int *)(b + l.incn);
     s.frag = (float4 *)(b + l.frag);
     s.frow = (int *)(b + l.frow);
+    s.fragd = (double4 *)(b + l.fragd);
     s.pair = (float4 *)(b + (l.pair >= 0 ? l.pair : 0));
     s.spt = (int4 *)(b + l.spt);
     s.misc = (int *)(b + l.misc);
@@ -310,9 +312,12 @@ __device__ inline float two_diff(float a, float b, float &e)
 
 // ---- pair energies of the buffered included groups ------------------------
 // lane <-> environment atom (float4 load, coalesced within a group), loop
-// over the fragment atoms held in LDS; FP32 terms, TwoSum into (acc, cmp).
+// over the fragment atoms held in LDS; FP64 terms into acc64 (default), or
+// FP32 terms with TwoSum into (acc, cmp) (fp32_coulomb and Ewald modes).
+// (cxd, cyd, czd): the trial centre; wrap: image shift per pair.
 template <int NT, bool STAGED, bool EW>
-__device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
+__device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64,
+                                double cxd, double cyd, double czd, bool wrap)
 {
     const int A = wg_scan<NT>(L.incn, n, L.misc, L.amap, L.map_cap);
     const bool mapped = A <= L.map_cap;
@@ -332,6 +337,42 @@ __device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc
         else
             return GPT[L.frow[f] + ty];
     };
+    // Default mode: every pair term in FP64.  The environment atom's
+    // position relative to the trial centre, image-shifted once per atom, and
+    // the fragment atom's (L.fragd) are exact differences of FP32 positions
+    // (to 2^-53); their difference needs no further image shift when
+    // Rep::nowrap holds, else it takes one per pair.  1/r: the FP32 estimate
+    // (relative error ~2^-22) and one FP64 Newton step (~1.5 2^-44).
+    const double Lxd = Lx, Lyd = Ly, Lzd = Lz, ixd = ix, iyd = iy, izd = iz;
+    auto atom_terms64 = [&](const float4 at, const int ty) {
+        double ax = (double)at.x - cxd, ay = (double)at.y - cyd, az = (double)at.z - czd;
+        ax = fma(-rint(ax * ixd), Lxd, ax);
+        ay = fma(-rint(ay * iyd), Lyd, ay);
+        az = fma(-rint(az * izd), Lzd, az);
+        const double qa = (double)at.w;
+        // the table entry of fragment atom f + 1 is fetched while atom f's
+        // terms are computed (the last fetch repeats atom nf - 1's)
+        float4 pn = pair_row(0, ty);
+        for (int f = 0; f < nf; ++f) {
+            const double4 F = L.fragd[f];
+            const float4 pp = pn;
+            pn = pair_row(min(f + 1, nf - 1), ty);
+            double ux = ax - F.x, uy = ay - F.y, uz = az - F.z;
+            if (wrap) {
+                ux = fma(-rint(ux * ixd), Lxd, ux);
+                uy = fma(-rint(uy * iyd), Lyd, uy);
+                uz = fma(-rint(uz * izd), Lzd, uz);
+            }
+            const double r2 = fma(uz, uz, fma(uy, uy, ux * ux));
+            double y = (double)__builtin_amdgcn_rsqf((float)r2);
+            y = y * fma(-0.5 * r2 * y, y, 1.5);
+            const double s2 = ((double)pp.x + (double)pp.z) * (y * y);   // 100 sigma^2 / r^2
+            const double s6 = s2 * s2 * s2;
+            accd = fma((double)pp.y, fma(s6, s6, -s6), accd);            // 4 eps (s^12 - s^6)
+            accd = fma(F.w * qa, y, accd);                                // K q_f q_a / r
+        }
+    };
+    // fp32_coulomb and Ewald modes: FP32 terms, TwoSum into (acc, cmp)
     auto atom_terms = [&](const float4 at, const int ty) {
         // the table entry of fragment atom f + 1 is fetched while atom f's
         // terms are computed (the last fetch repeats atom nf - 1's)
@@ -363,36 +404,23 @@ __device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc
             s2 = fmaf(fmaf(-s2, r2, pp.x) + pp.z, rr, s2);
             const float s6 = s2 * s2 * s2;
             float lj = pp.y * fmaf(s6, s6, -s6);           // 4 eps (s^12 - s^6)
-            float el = 0.0f;
-            if (s2 > 1.0f || c64) {
-                // FP64 from the exact displacement (s + e per axis); FP64 FMA
-                // runs at the FP32 rate on this GPU
+            if (s2 > 1.0f) {
+                // inside sigma s^12 dominates and carries six times the
+                // relative error of r^2: this rare pair's LJ term in FP64,
+                // from the exact displacement
                 const double ux = (double)dx + (double)ex, uy = (double)dy + (double)ey,
                              uz = (double)dz + (double)ez;
                 const double r2d = fma(uz, uz, fma(uy, uy, ux * ux));
-                if (s2 > 1.0f) {
-                    // inside sigma s^12 dominates and carries six times the
-                    // relative error of r^2: this rare pair's LJ term in FP64
-                    double q = __builtin_amdgcn_rcp(r2d);
-                    q = q * fma(-r2d, q, 2.0);
-                    q = q * fma(-r2d, q, 2.0);
-                    const double s2d = ((double)pp.x + (double)pp.z) * q;
-                    const double s6d = s2d * s2d * s2d;
-                    accd = fma((double)pp.y, fma(s6d, s6d, -s6d), accd);
-                    lj = 0.0f;
-                }
-                if (c64) {
-                    // K q_f q_a / r
-                    double y = __builtin_amdgcn_rsq(r2d);
-                    y = y * fma(-0.5 * r2d * y, y, 1.5);
-                    y = y * fma(-0.5 * r2d * y, y, 1.5);
-                    accd = fma((double)fr.w * (double)at.w, y, accd);
-                }
+                double q = __builtin_amdgcn_rcp(r2d);
+                q = q * fma(-r2d, q, 2.0);
+                q = q * fma(-r2d, q, 2.0);
+                const double s2d = ((double)pp.x + (double)pp.z) * q;
+                const double s6d = s2d * s2d * s2d;
+                accd = fma((double)pp.y, fma(s6d, s6d, -s6d), accd);
+                lj = 0.0f;
             }
-            if (!c64) {
-                el = (fr.w * at.w) * ir;                    // K q_f q_a / r
-                if (EW) el *= erfcf(eb * (r2 * ir));        // Ewald real-space part
-            }
+            float el = (fr.w * at.w) * ir;                  // K q_f q_a / r
+            if (EW) el *= erfcf(eb * (r2 * ir));            // Ewald real-space part
             const float term = lj + el;
             const float s = acc + term;                     // TwoSum
             const float bp = s - acc;
@@ -418,8 +446,13 @@ __device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc
         const int ty0 = atype[a0];
         const float4 at1 = atom[a1];
         const int ty1 = atype[a1];
-        atom_terms(at0, ty0);
-        if (two) atom_terms(at1, ty1);
+        if (c64) {
+            atom_terms64(at0, ty0);
+            if (two) atom_terms64(at1, ty1);
+        } else {
+            atom_terms(at0, ty0);
+            if (two) atom_terms(at1, ty1);
+        }
     }
     __syncthreads();
     acc64 += accd;
@@ -429,10 +462,12 @@ __device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc
 // the energy pass of eval_energy
 template <bool EW>
 struct EnergyPass {
+    double cx, cy, cz;   // trial centre
+    bool wrap;           // image shift per pair (else per environment atom)
     template <int NT, bool STAGED>
     __device__ int run(CRep &C, const Lds &L, int n, int nf, float &acc, float &cmp, double &acc64)
     {
-        return pair_pass<NT, STAGED, EW>(C, L, n, nf, acc, cmp, acc64);
+        return pair_pass<NT, STAGED, EW>(C, L, n, nf, acc, cmp, acc64, cx, cy, cz, wrap);
     }
 };
 
@@ -445,14 +480,17 @@ __device__ inline void load_species_table(CRep &C, const Lds &L)
     }
 }
 
-// pair-table rows of species spi's distinct atom types into LDS (atom f's
-// row at L.frow[f] = F.urow[f] when staged)
+// the replica's pair table (X x Y entries) into LDS, straight from global
+// memory (global_load_lds: no registers); the caller waits (vmcnt) and
+// synchronises before the first read
 template <int NT>
-__device__ inline void stage_species_rows(CRep &C, int spi, int nu, const Lds &L)
+__device__ inline void stage_pair_table(CRep &C, const Lds &L)
 {
-    const auto SP = gbl(C.spair) + (size_t)spi * MAXF * C.Y;
-    const int n = nu * C.Y;
-    for (int i = threadIdx.x; i < n; i += NT) L.pair[i] = SP[i];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = C.X * C.Y;
+    const auto P = gbl(C.pair);
+    for (int b = wave * 64; b < n; b += NT)
+        if (b + lane < n) __builtin_amdgcn_global_load_lds((const void *)(P + b + lane), (void *)&L.pair[b], 16, 0, 0);
 }
 
 // ---- calcEnergy over the cell list -----------------------------------------
@@ -593,7 +631,7 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(CRep &C, 
             cnt += __popcll(bal);
         }
         for (int q = cnt + lane; q < SEG; q += 64) L.incn[wave * SEG + q] = 0;
-        // staged pair rows were loaded into LDS asynchronously (eval_item):
+        // a staged pair table was loaded into LDS asynchronously (eval_item):
         // every wave's copies have landed once the barrier passes
         if (STAGED) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
@@ -633,13 +671,25 @@ __device__ __attribute__((always_inline)) inline void eval_energy_pass(CRep &C, 
     __syncthreads();
 }
 
+// pose_near: the pose in L.frag was generated from the species template or
+// the molecule's atoms (within the species' extent of the centre, the bound
+// Rep::nowrap rests on); poses given by a caller take the per-pair image shift.
+// Thread f reads L.frag[f] (written by thread f, or before a barrier) and
+// writes L.fragd[f]; eval_energy_pass has a barrier before its first pair
+// pass.
 template <int NT, bool STAGED, bool EW = false>
 __device__ __attribute__((always_inline)) inline void eval_energy(CRep &C, float cx, float cy, float cz, int nf,
                                                                   int exclude, const Lds &L, double *outE, int *outN,
                                                                   uint32_t *outH, int *outA,
-                                                                  unsigned long long *ticks = nullptr)
+                                                                  unsigned long long *ticks = nullptr,
+                                                                  bool pose_near = false)
 {
-    EnergyPass<EW> P;
+    EnergyPass<EW> P{(double)cx, (double)cy, (double)cz, !(pose_near && C.nowrap != 0)};
+    const int tid = threadIdx.x;
+    if (tid < nf) {
+        const float4 q = L.frag[tid];
+        L.fragd[tid] = make_double4((double)q.x - P.cx, (double)q.y - P.cy, (double)q.z - P.cz, (double)q.w);
+    }
     eval_energy_pass<NT, STAGED>(C, cx, cy, cz, nf, exclude, L, P, outE, outN, outH, outA, ticks);
 }
 
@@ -735,7 +785,7 @@ template <int NT, bool STAGED, bool LOCAL, bool EW>
 // Thread 0 adds the statistics to S, or leaves them in *defer (thread 0's
 // copy) for the caller to add once the item's completion is published.
 __device__ inline void eval_item(CRep &C, const State &S, int item, const Lds &L, bool stage_rep = true,
-                                 bool stage_rows = true, ItemStats *defer = nullptr)
+                                 bool stage_tab = true, ItemStats *defer = nullptr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long tq0 = wall_clock64();
@@ -750,20 +800,13 @@ __device__ inline void eval_item(CRep &C, const State &S, int item, const Lds &L
         phase = 1;
     }
     // the species table stays in LDS while a workgroup keeps evaluating
-    // items of one replica (stage_rep = false), the species' pair rows while
-    // they are also of one species (stage_rows = false)
+    // items of one replica (stage_rep = false), the pair table while it
+    // keeps evaluating items of replicas with that table (stage_tab = false)
     if (stage_rep) load_species_table(C, L);
-    // the species' pair rows go from global memory straight into LDS
-    // (global_load_lds: no registers, no wait here); the pose, stencil and
+    // the pair table is copied without a wait here: the pose, stencil and
     // candidate work runs meanwhile, and the wait is before the first pair
     // pass (eval_energy_pass)
-    const bool stg = STAGED && stage_rows;
-    if (stg) {
-        const int nrow = F.nu * C.Y;
-        const auto SP = gbl(C.spair) + (size_t)spi * MAXF * C.Y;
-        for (int b = wave * 64; b < nrow; b += NT)
-            if (b + lane < nrow) __builtin_amdgcn_global_load_lds((const void *)(SP + b + lane), (void *)&L.pair[b], 16, 0, 0);
-    }
+    if (STAGED && stage_tab) stage_pair_table<NT>(C, L);
     const float4 tcv = tc[t];
     const int slot = (kind == K_INS) ? -1 : gbl(C.tslot)[t];
     const int gid = (kind == K_INS) ? -1 : F.startRes + slot;
@@ -824,7 +867,7 @@ __device__ inline void eval_item(CRep &C, const State &S, int item, const Lds &L
             }
         }
         L.frag[a] = make_float4(px, py, pz, F.kq[a]);
-        L.frow[a] = STAGED ? F.urow[a] : F.trow[a];
+        L.frow[a] = F.trow[a];
         if (kind == K_INS || phase == 1) gbl(C.pose)[t * MAXF + a] = make_float4(px, py, pz, 0.f);
     }
     __syncthreads();
@@ -852,7 +895,7 @@ __device__ inline void eval_item(CRep &C, const State &S, int item, const Lds &L
     uint32_t hh;
     unsigned long long ticks[4];
     ticks[0] = wall_clock64() - tq0;
-    eval_energy<NT, STAGED, EW>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat, ticks);
+    eval_energy<NT, STAGED, EW>(C, cx, cy, cz, nf, gid, L, &E, &nin, &hh, &nat, ticks, true);
     if (EW) {
         // the moving molecule (delete, translate, rotate) is in the PME grid
         const double er = ewald_recip<NT>(C, F, L, nf, kind == K_INS ? -1 : F.atomBase + slot * nf);
@@ -1657,9 +1700,8 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
     if (nc <= 0) return;
     const int rot = (int)(blockIdx.x / (unsigned)xm.nclass) % nc;
     int idle = 0, seen = -1;
-    // replica whose species table, and replica * MAXT + species whose pair
-    // rows, are in LDS
-    int cached_rep = -1, cached_rows = -1;
+    // replica whose species table, and pair table (Rep::tab) that are in LDS
+    int cached_rep = -1, cached_tab = -1;
     while (true) {
         if (tid < 64) {
             // wave 0 polls every replica of this XCD at once, one claim word
@@ -1705,7 +1747,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
                     // affinity (aff > 0): a move of at least aff unclaimed items
                     // of the replica whose tables this workgroup holds in LDS
                     // comes before the other such moves (its species table and
-                    // staged pair rows are reused)
+                    // staged pair table are reused)
                     if (aff > 0 && unc >= (unsigned long long)aff) size = (r == cached_rep) ? 2u : 3u;
                     const unsigned prio = (unsigned)(w >> 40) >> 8;
                     while (__ballot(avail) != 0ull) {
@@ -1755,11 +1797,11 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
         CRep &C = rep_of(reps, found);
         State &S = st[found];
         const unsigned long long te0 = wall_clock64();
-        const int rows = found * MAXT + S.sp;
+        const int tab = C.tab;
         ItemStats ist;
-        eval_item<NT, STAGED, true, false>(C, S, item, L, found != cached_rep, rows != cached_rows, &ist);
+        eval_item<NT, STAGED, true, false>(C, S, item, L, found != cached_rep, tab != cached_tab, &ist);
         cached_rep = found;
-        cached_rows = rows;
+        cached_tab = tab;
         const unsigned long long te1 = wall_clock64();
         xcd_release();
         __syncthreads();
@@ -1802,7 +1844,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
                 __hip_atomic_fetch_add(&sch->progress, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             __builtin_amdgcn_s_setprio(0);
-            cached_rep = cached_rows = -1;  // accept and prep reuse the evaluation region
+            cached_rep = cached_tab = -1;  // accept and prep reuse the evaluation region
         }
         __syncthreads();
     }
@@ -1825,12 +1867,12 @@ __global__ __launch_bounds__(NT) void k_eval_poses(const Rep *reps, int r, int s
     CRep &C = rep_of(reps, r);
     CSpecies &F = species_of(C, spi);
     const int nf = F.nf, i = blockIdx.x, tid = threadIdx.x;
-    if (STAGED) stage_species_rows<NT>(C, spi, F.nu, L);
+    if (STAGED) stage_pair_table<NT>(C, L);
     load_species_table(C, L);
     if (tid < nf) {
         const float *p = poses + ((size_t)i * nf + tid) * 3;
         L.frag[tid] = make_float4(p[0], p[1], p[2], F.kq[tid]);
-        L.frow[tid] = STAGED ? F.urow[tid] : F.trow[tid];
+        L.frow[tid] = F.trow[tid];
     }
     __syncthreads();
     double e;

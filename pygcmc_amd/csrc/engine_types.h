@@ -44,9 +44,6 @@ struct Species {
     float kq[MAXF];        // 1388.431112 * charge, rounded once
     int trow[MAXF];        // type * Y: row of the pair table
     int ttype[MAXF];       // type
-    int urow[MAXF];        // u * Y: row of the atom's type in the species' staged rows
-    int nu;                // distinct atom types of the species (rows staged in LDS)
-    int pad2[3];
 };
 
 struct DevTrace {
@@ -84,7 +81,12 @@ struct Rep {
     float cell_h[3];         // cell edges L / nc
     float prune_r2;          // (rc + stencil slack)^2
     uint32_t seed;
-    int pad0;
+    int nowrap;              // pair displacements need no per-pair image shift: every environment
+                             // atom of an included group and every fragment atom lie, per axis,
+                             // less than L/2 apart after one image shift of the atom around the
+                             // trial centre (rc + group extents < L/2, host-checked)
+    int tab;                 // replicas with equal tab have equal pair tables (one copy in LDS serves them)
+    int pad1;
     const float *grid;
     const int *moves;
     const uint32_t *wcum;    // [nmoves+1] estimated critical-path cost of moves [0, s), scaled (scheduling)
@@ -93,7 +95,6 @@ struct Rep {
     float4 *atom;
     int *atype;
     const float4 *pair;      // (100 sigma^2 hi, 4 eps, 100 sigma^2 lo, 0) per (fragment type, type)
-    const float4 *spair;     // [T][MAXF][Y]: pair rows of each species' distinct atom types, contiguous
     const Species *sp;
     // per cell c in [0, ncell]: (first fixed record, mobile count); the fixed
     // records of cell c are [cell[c].x, cell[c + 1].x), and cell[ncell].y
@@ -154,7 +155,8 @@ struct LdsLayout {
     int smob, sfst, sfix, soff;    // stencil entries (int[nst+2]): mobile-slot base, fixed start, fixed count, offset
     int incs, incn;                // included groups: first atom, atom count (int[inc_cap+1])
     int frag, frow;                // fragment pose (float4[MAXF], int[MAXF])
-    int pair;                      // float4[nf*Y] (one species' rows) or -1
+    int fragd;                     // fragment pose relative to the trial centre, FP64 (double4[MAXF])
+    int pair;                      // float4[X*Y]: the pair table held in LDS, or -1
     int spt;                       // species table int4[MAXT] (startRes, maxNum, atomBase, nf)
     int misc, dred;                // int[MISC_INTS], double[MAXWAVE]
     int members, w, val, accl;     // accept: int[maxk], double[maxk] x2, int2[maxk]

@@ -158,7 +158,7 @@ def test_plain_batched_graph_identical(wg_size):
 
 
 def test_pair_rows_in_lds_identical():
-    """Pair-table rows staged in LDS (copied with global_load_lds) or read
+    """The pair table held in LDS (copied with global_load_lds) or read
     from global memory: the same values, so bit-identical energies and
     chains, in the batched and plain forms."""
     s = systems.multi(nsteps=300, seed=13)
