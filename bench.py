@@ -13,9 +13,12 @@ its initial state on the device (device-to-device copies from a snapshot
 taken after upload), then all 10,000 moves of all 64 systems (640,000 MC
 steps).  --warmup passes are untimed, --steps passes are timed; every pass
 computes the same chains, and the bench checks that they end in the same
-state.  The engine assigns systems to XCDs by the work it measured in the
-previous run (the first pass by its cost estimate), so passes after the
-first run balanced; --xcd-round-robin 1 keeps the index order.
+state.  The engine assigns systems to XCDs by its cost estimate for the
+first pass and by the work measured in it for the second; the next three
+passes each try exchanging two systems between the XCDs that finished last
+and first, and the fastest assignment serves every pass after them (with
+--warmup 5, every timed pass).  --xcd-search -1 stops after the second
+pass's balance; --xcd-round-robin 1 keeps the index order.
 
 Prints one JSON line.  `value` is the batched form's aggregate MC steps/s over
 all systems with inputs resident in HBM.  Beside it: the plain form (one move
@@ -395,6 +398,8 @@ def main():
     ap.add_argument('--pair-lds', type=int, default=0)
     ap.add_argument('--fp32-coulomb', type=int, default=0)
     ap.add_argument('--xcd-round-robin', type=int, default=0)
+    ap.add_argument('--xcd-search', type=int, default=0,
+                    help='passes that may try an exchange of two systems between XCDs (0 = 3, -1 = none)')
     ap.add_argument('--kernel-only', action='store_true',
                     help='only the timed passes (for profiler runs): no graph, plain, lock-step, '
                          'adjoint, set-up or CPU legs')
@@ -429,7 +434,7 @@ def main():
     R = len(systems)
     log(f'built {R} systems ({int(sizes.sum())} fixed atoms, {M} moves each) in {time.time() - t0:.1f}s')
     tuning = dict(cell_size=args.cell_size, persistent_wgs=args.wgs, wg_size=args.wg_size, pair_lds=args.pair_lds,
-                  fp32_coulomb=args.fp32_coulomb, xcd_round_robin=args.xcd_round_robin)
+                  fp32_coulomb=args.fp32_coulomb, xcd_round_robin=args.xcd_round_robin, xcd_search=args.xcd_search)
     tune_table = None
     if args.autotune:
         from pygcmc_amd.autotune import autotune
@@ -437,6 +442,7 @@ def main():
         tuning, tune_table = autotune(systems, log=log)
         tuning['fp32_coulomb'] = args.fp32_coulomb
         tuning['xcd_round_robin'] = args.xcd_round_robin
+        tuning['xcd_search'] = args.xcd_search
         log(f'autotune: {tuning} in {time.time() - t1:.1f}s')
 
     eng = make_engine(copies(systems), tuning)

@@ -104,6 +104,13 @@ typedef struct gcmc_tuning {
                             balanced, heaviest first to the least loaded XCD, by estimated work
                             before the first run and by the work measured in the last run after
                             it.  Changes which workgroups serve a system, never results */
+    int xcd_search;      /* batched form, balanced: when runs repeat one step range (the
+                            snapshot restored between them), up to this many of them each try
+                            exchanging two systems between the XCD that finished last and the
+                            one that finished first under the fastest assignment so far, chosen
+                            from the systems' measured evaluation times; the fastest assignment
+                            is kept.  0 (default) = 3 tries, -1 = none.  Changes which workgroups
+                            serve a system, never results */
 } gcmc_tuning;
 
 #define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */

@@ -172,6 +172,19 @@ struct gcmc_engine {
     int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]
     State *d_state_snap = nullptr;  // State of every replica at gcmc_engine_snapshot
     std::vector<int> cls_host;      // the replica -> XCD class map on the device (cls_start, cls_reps)
+    std::vector<int> cls_of;        // the same, class of each replica
+    // Search over XCD assignments while runs repeat one step range (tuning
+    // xcd_search): the fastest assignment so far with its launch time, the
+    // per-class finish times and per-replica evaluation times measured under
+    // it, the exchanges already tried from it, and the assignment of the next run
+    struct XcdSearch {
+        std::vector<int> range;     // (first step, end) per replica of the runs compared
+        std::vector<int> best, next;
+        double best_ms = 0.0;
+        std::vector<double> fin, evt;
+        std::vector<std::pair<int, int>> tried;
+        int runs = 0;
+    } xs;
     double *d_adj_work = nullptr;  // column-sum partials of the adjoint backward
     size_t adj_work_n = 0;
 };
@@ -775,6 +788,25 @@ struct RunPoses {
     }
 };
 
+// The class of each replica to the device (cls_start, cls_reps).
+int upload_classes(gcmc_engine *E, const std::vector<int> &cls_of)
+{
+    XcdMap &m = E->xm;
+    const int R = (int)E->reps.size();
+    std::vector<int> cls(m.nclass + 1 + R, 0);
+    int o = 0;
+    for (int c = 0; c < m.nclass; ++c) {
+        cls[c] = o;
+        for (int r = 0; r < R; ++r)
+            if (cls_of[r] == c) cls[m.nclass + 1 + o++] = r;
+    }
+    cls[m.nclass] = o;
+    HIPX(hipMemcpy(E->d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice));
+    E->cls_host = cls;
+    E->cls_of = cls_of;
+    return 0;
+}
+
 // Replicas to XCD classes (each replica is served by the workgroups of one
 // XCD).  Round-robin when the tuning asks for it; otherwise the heaviest
 // replica first to the class with the least work so far (longest processing
@@ -782,37 +814,76 @@ struct RunPoses {
 // before the first run and the measured work of the last run after it: the
 // run ends when its slowest XCD does, and the XCDs' loads then differ least.
 // Changes which workgroups evaluate a replica's items, never the results.
-int assign_classes(gcmc_engine *E, const std::vector<double> &w)
+std::vector<int> lpt_classes(const gcmc_engine *E, const std::vector<double> &w)
 {
-    XcdMap &m = E->xm;
-    const int R = (int)E->reps.size();
-    std::vector<std::vector<int>> members(m.nclass);
+    const int nclass = E->xm.nclass, R = (int)E->reps.size();
+    std::vector<int> cls_of(R, 0);
     if (E->tune.xcd_round_robin) {
-        for (int r = 0; r < R; ++r) members[r % m.nclass].push_back(r);
-    } else {
-        std::vector<int> order(R);
-        for (int r = 0; r < R; ++r) order[r] = r;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
-        std::vector<double> load(m.nclass, 0.0);
-        for (int r : order) {
-            int c = 0;
-            for (int q = 1; q < m.nclass; ++q)
-                if (load[q] < load[c]) c = q;
-            load[c] += w[r];
-            members[c].push_back(r);
+        for (int r = 0; r < R; ++r) cls_of[r] = r % nclass;
+        return cls_of;
+    }
+    std::vector<int> order(R);
+    for (int r = 0; r < R; ++r) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+    std::vector<double> load(nclass, 0.0);
+    for (int r : order) {
+        int c = 0;
+        for (int q = 1; q < nclass; ++q)
+            if (load[q] < load[c]) c = q;
+        load[c] += w[r];
+        cls_of[r] = c;
+    }
+    return cls_of;
+}
+
+int assign_classes(gcmc_engine *E, const std::vector<double> &w) { return upload_classes(E, lpt_classes(E, w)); }
+
+// One exchange to try from the fastest assignment so far: a replica a of the
+// class that finished last for a replica b of the class that finished first,
+// the pair whose evaluation times (each class's finish time taken as
+// proportional to its summed evaluation time) leave the later of the two
+// classes' predicted finishes earliest; none when no untried pair would end
+// the slower class sooner.
+bool propose_exchange(const gcmc_engine *E, std::vector<int> &next, std::pair<int, int> &pair)
+{
+    const gcmc_engine::XcdSearch &X = E->xs;
+    const int nclass = E->xm.nclass, R = (int)X.best.size();
+    int S = 0, F = 0;
+    for (int c = 1; c < nclass; ++c) {
+        if (X.fin[c] > X.fin[S]) S = c;
+        if (X.fin[c] < X.fin[F]) F = c;
+    }
+    if (S == F) return false;
+    double eS = 0.0, eF = 0.0;
+    for (int r = 0; r < R; ++r) {
+        if (X.best[r] == S) eS += X.evt[r];
+        if (X.best[r] == F) eF += X.evt[r];
+    }
+    if (eS <= 0.0 || eF <= 0.0) return false;
+    const double kS = X.fin[S] / eS, kF = X.fin[F] / eF;
+    double top = X.fin[S];
+    int ba = -1, bb = -1;
+    for (int a = 0; a < R; ++a) {
+        if (X.best[a] != S) continue;
+        for (int b = 0; b < R; ++b) {
+            if (X.best[b] != F) continue;
+            const double d = X.evt[a] - X.evt[b];
+            if (d <= 0.0) continue;
+            if (std::find(X.tried.begin(), X.tried.end(), std::make_pair(a, b)) != X.tried.end()) continue;
+            const double pred = std::max(X.fin[S] - kS * d, X.fin[F] + kF * d);
+            if (pred < top) {
+                top = pred;
+                ba = a;
+                bb = b;
+            }
         }
-        for (auto &v : members) std::sort(v.begin(), v.end());
     }
-    std::vector<int> cls(m.nclass + 1 + R, 0);
-    int o = 0;
-    for (int c = 0; c < m.nclass; ++c) {
-        cls[c] = o;
-        for (int r : members[c]) cls[m.nclass + 1 + o++] = r;
-    }
-    cls[m.nclass] = o;
-    HIPX(hipMemcpy(E->d_cls, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice));
-    E->cls_host = cls;
-    return 0;
+    if (ba < 0) return false;
+    next = X.best;
+    next[ba] = F;
+    next[bb] = S;
+    pair = std::make_pair(ba, bb);
+    return true;
 }
 
 // Which XCC ids the device's workgroups run on, and the assignment of
@@ -1048,18 +1119,39 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
                                 "or none; run mixed engines in plain mode");
     if (mode != GCMC_MODE_PLAIN && mode != GCMC_MODE_BATCHED && mode != GCMC_MODE_GRAPH)
         return fail(GCMC_E_ARG, "unknown run mode");
-    if (mode != GCMC_MODE_PLAIN && n_ew == 0 && !E->tune.xcd_round_robin) {
-        // balance the XCDs by the work each replica did in the last run
-        std::vector<double> w(E->reps.size());
-        bool measured = false;
-        for (size_t r = 0; r < w.size(); ++r) {
-            w[r] = E->reps[r]->last_work;
-            measured |= w[r] > 0.0;
+    const bool balance = mode != GCMC_MODE_PLAIN && n_ew == 0 && !E->tune.xcd_round_robin;
+    const int R = (int)E->reps.size();
+    std::vector<State> st0;
+    if (balance) {
+        gcmc_engine::XcdSearch &X = E->xs;
+        std::vector<int> range(2 * R);
+        for (int r = 0; r < R; ++r) {
+            const HostRep &H = *E->reps[r];
+            range[2 * r] = H.host_step;
+            range[2 * r + 1] = (int)std::min<long long>((long long)H.host_step + nsteps, H.rep.nmoves);
         }
-        if (measured) {
-            int rc = assign_classes(E, w);
+        if (X.range == range && !X.next.empty()) {
+            // the same steps again: the assignment chosen after the last run
+            int rc = upload_classes(E, X.next);
             if (rc) return rc;
+        } else {
+            // other steps: balance the XCDs by the work each replica did in
+            // the last run (else keep the estimate's assignment)
+            X = gcmc_engine::XcdSearch{};
+            X.range = range;
+            std::vector<double> w(R);
+            bool measured = false;
+            for (int r = 0; r < R; ++r) {
+                w[r] = E->reps[r]->last_work;
+                measured |= w[r] > 0.0;
+            }
+            if (measured) {
+                int rc = assign_classes(E, w);
+                if (rc) return rc;
+            }
         }
+        int rc = read_states(E, st0);
+        if (rc) return rc;
     }
     std::vector<int> step0(E->reps.size());
     for (size_t r = 0; r < step0.size(); ++r) step0[r] = E->reps[r]->host_step;
@@ -1132,6 +1224,44 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
             return fail(GCMC_E_DEVICE, "replica " + std::to_string(r) + " stopped at step " +
                                            std::to_string(st[r].step) + " before its end " +
                                            std::to_string(st[r].end));
+    }
+    if (balance) {
+        // this run's time, per-replica evaluation time (workgroup-seconds) and
+        // per-class finish (the latest end of a replica's chain, which runs
+        // from its first publication through its evaluations, accepts and
+        // preps: 100 MHz ticks)
+        gcmc_engine::XcdSearch &X = E->xs;
+        const int nclass = E->xm.nclass;
+        const double t = mode == GCMC_MODE_BATCHED ? E->kernel_ms : E->last_ms;
+        std::vector<double> evt(R), fin(nclass, 0.0);
+        for (int r = 0; r < R; ++r) {
+            evt[r] = (double)(st[r].t_eval - st0[r].t_eval) * 1e-8;
+            const unsigned long long el = (st[r].t_step - st0[r].t_step) + (st[r].t_accept - st0[r].t_accept) +
+                                          (st[r].t_prep - st0[r].t_prep);
+            const int c = E->cls_of[r];
+            fin[c] = std::max(fin[c], (double)el * 1e-5);
+        }
+        X.runs += 1;
+        if (X.best.empty() || t < X.best_ms) {
+            X.best = E->cls_of;
+            X.best_ms = t;
+            X.fin = fin;
+            X.evt = evt;
+            X.tried.clear();
+        }
+        const int tries = E->tune.xcd_search == 0 ? 3 : std::max(E->tune.xcd_search, 0);
+        std::vector<double> w(R);
+        for (int r = 0; r < R; ++r) w[r] = E->reps[r]->last_work;
+        std::pair<int, int> pr;
+        if (X.runs == 1) {
+            // first run of these steps (on the estimate's or an earlier range's
+            // assignment): next, the balance by this run's measured work
+            X.next = lpt_classes(E, w);
+        } else if (X.runs - 1 <= tries && propose_exchange(E, X.next, pr)) {
+            X.tried.push_back(pr);
+        } else {
+            X.next = X.best;
+        }
     }
     return 0;
 }

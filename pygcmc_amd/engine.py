@@ -34,7 +34,8 @@ _lib = None
 class Tuning(ctypes.Structure):
     _fields_ = [('cell_size', ctypes.c_float), ('persistent_wgs', ctypes.c_int),
                 ('bucket_cap', ctypes.c_int), ('pair_lds', ctypes.c_int), ('wg_size', ctypes.c_int),
-                ('fp32_coulomb', ctypes.c_int), ('claim_affinity', ctypes.c_int), ('xcd_round_robin', ctypes.c_int)]
+                ('fp32_coulomb', ctypes.c_int), ('claim_affinity', ctypes.c_int), ('xcd_round_robin', ctypes.c_int),
+                ('xcd_search', ctypes.c_int)]
 
 
 class CTrace(ctypes.Structure):
@@ -140,10 +141,10 @@ class Engine:
     """Independent GCMC systems on one GPU (see module docstring)."""
 
     def __init__(self, cell_size=0.0, persistent_wgs=0, bucket_cap=0, pair_lds=0, wg_size=0, fp32_coulomb=0,
-                 claim_affinity=0, xcd_round_robin=0):
+                 claim_affinity=0, xcd_round_robin=0, xcd_search=0):
         self._h = ctypes.c_void_p()
         t = Tuning(cell_size, persistent_wgs, bucket_cap, pair_lds, wg_size, fp32_coulomb, claim_affinity,
-                   xcd_round_robin)
+                   xcd_round_robin, xcd_search)
         _check(lib().gcmc_engine_create(ctypes.byref(self._h), ctypes.byref(t)))
         self.systems = []
         self._traces = {}

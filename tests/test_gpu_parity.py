@@ -280,9 +280,9 @@ def test_snapshot_restore_reruns_identical_chain():
 
 
 def test_xcd_assignment_never_changes_chains():
-    """Systems assigned to XCDs round-robin, by estimated work, or by the work
-    measured in the previous run (the second run of a snapshot): the same
-    final states."""
+    """Systems assigned to XCDs round-robin, by estimated work, by the work
+    measured in the previous run (the second run of a snapshot), or by the
+    exchanges tried on later runs of the same steps: the same final states."""
     base = [systems.multi(nsteps=200, n_fixed=n, box=b, seed=sd, conf_scale=0.2)
             for n, b, sd in ((900, 28.0, 41), (2400, 44.0, 42), (1500, 36.0, 43), (1200, 33.0, 44),
                              (2000, 42.0, 45), (1000, 30.0, 46), (1800, 40.0, 47), (1300, 34.0, 48),
@@ -295,7 +295,9 @@ def test_xcd_assignment_never_changes_chains():
             eng.add_system(g)
         eng.upload()
         eng.snapshot()
-        for _ in range(2):          # the second run is balanced by the first's measured work
+        # the second run is balanced by the first's measured work; the third
+        # to fifth try exchanges of two systems between XCDs
+        for _ in range(5):
             eng.restore()
             eng.run(200, 'batched')
             out = []
