@@ -15,8 +15,8 @@ steps).  --warmup passes are untimed, --steps passes are timed; every pass
 computes the same chains, and the bench checks that they end in the same
 state.  The engine assigns systems to XCDs by its cost estimate for the
 first pass and by the work measured in it for the second; the next three
-passes each try exchanging two systems between the XCDs that finished last
-and first, and the fastest assignment serves every pass after them (with
+passes each try exchanges of systems between XCDs that finished late and
+early, and the fastest assignment serves every pass after them (with
 --warmup 5, every timed pass).  --xcd-search -1 stops after the second
 pass's balance; --xcd-round-robin 1 keeps the index order.
 

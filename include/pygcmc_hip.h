@@ -106,11 +106,12 @@ typedef struct gcmc_tuning {
                             it.  Changes which workgroups serve a system, never results */
     int xcd_search;      /* batched form, balanced: when runs repeat one step range (the
                             snapshot restored between them), up to this many of them each try
-                            exchanging two systems between the XCD that finished last and the
-                            one that finished first under the fastest assignment so far, chosen
-                            from the systems' measured evaluation times; the fastest assignment
-                            is kept.  0 (default) = 3 tries, -1 = none.  Changes which workgroups
-                            serve a system, never results */
+                            exchanges of systems between XCDs of the fastest assignment so far
+                            (the i-th XCD to finish last with the i-th to finish first, one
+                            exchange of two systems per pair, chosen from the systems' measured
+                            evaluation times); the fastest assignment is kept.  0 (default) = 3
+                            tries, -1 = none.  Changes which workgroups serve a system, never
+                            results */
 } gcmc_tuning;
 
 #define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */

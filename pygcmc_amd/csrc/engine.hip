@@ -838,22 +838,16 @@ std::vector<int> lpt_classes(const gcmc_engine *E, const std::vector<double> &w)
 
 int assign_classes(gcmc_engine *E, const std::vector<double> &w) { return upload_classes(E, lpt_classes(E, w)); }
 
-// One exchange to try from the fastest assignment so far: a replica a of the
-// class that finished last for a replica b of the class that finished first,
-// the pair whose evaluation times (each class's finish time taken as
+// One exchange between class S (finished later) and class F under the
+// fastest assignment so far, applied to next: a replica a of S for a replica
+// b of F, the pair whose evaluation times (each class's finish time taken as
 // proportional to its summed evaluation time) leave the later of the two
 // classes' predicted finishes earliest; none when no untried pair would end
-// the slower class sooner.
-bool propose_exchange(const gcmc_engine *E, std::vector<int> &next, std::pair<int, int> &pair)
+// S sooner.
+bool propose_one(const gcmc_engine *E, int S, int F, std::vector<int> &next, std::pair<int, int> &pair)
 {
     const gcmc_engine::XcdSearch &X = E->xs;
-    const int nclass = E->xm.nclass, R = (int)X.best.size();
-    int S = 0, F = 0;
-    for (int c = 1; c < nclass; ++c) {
-        if (X.fin[c] > X.fin[S]) S = c;
-        if (X.fin[c] < X.fin[F]) F = c;
-    }
-    if (S == F) return false;
+    const int R = (int)X.best.size();
     double eS = 0.0, eF = 0.0;
     for (int r = 0; r < R; ++r) {
         if (X.best[r] == S) eS += X.evt[r];
@@ -879,11 +873,32 @@ bool propose_exchange(const gcmc_engine *E, std::vector<int> &next, std::pair<in
         }
     }
     if (ba < 0) return false;
-    next = X.best;
     next[ba] = F;
     next[bb] = S;
     pair = std::make_pair(ba, bb);
     return true;
+}
+
+// The exchanges of one try from the fastest assignment so far: the classes
+// ordered by finish time, the i-th latest paired with the i-th earliest, one
+// exchange per pair (propose_one).  A try changes several classes at once;
+// the run measures it as a whole and the faster assignment is kept.
+bool propose_exchange(const gcmc_engine *E, std::vector<int> &next, std::vector<std::pair<int, int>> &pairs)
+{
+    const gcmc_engine::XcdSearch &X = E->xs;
+    const int nclass = E->xm.nclass;
+    std::vector<int> order(nclass);
+    for (int c = 0; c < nclass; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return X.fin[a] > X.fin[b]; });
+    next = X.best;
+    pairs.clear();
+    for (int i = 0; i < nclass / 2; ++i) {
+        const int S = order[i], F = order[nclass - 1 - i];
+        if (!(X.fin[S] > X.fin[F])) break;
+        std::pair<int, int> pr;
+        if (propose_one(E, S, F, next, pr)) pairs.push_back(pr);
+    }
+    return !pairs.empty();
 }
 
 // Which XCC ids the device's workgroups run on, and the assignment of
@@ -1252,13 +1267,13 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
         const int tries = E->tune.xcd_search == 0 ? 3 : std::max(E->tune.xcd_search, 0);
         std::vector<double> w(R);
         for (int r = 0; r < R; ++r) w[r] = E->reps[r]->last_work;
-        std::pair<int, int> pr;
+        std::vector<std::pair<int, int>> pr;
         if (X.runs == 1) {
             // first run of these steps (on the estimate's or an earlier range's
             // assignment): next, the balance by this run's measured work
             X.next = lpt_classes(E, w);
         } else if (X.runs - 1 <= tries && propose_exchange(E, X.next, pr)) {
-            X.tried.push_back(pr);
+            X.tried.insert(X.tried.end(), pr.begin(), pr.end());
         } else {
             X.next = X.best;
         }
