@@ -14,7 +14,7 @@ taken after upload), then all 10,000 moves of all 64 systems (640,000 MC
 steps).  --warmup passes are untimed, --steps passes are timed; every pass
 computes the same chains, and the bench checks that they end in the same
 state.  The engine assigns systems to XCDs by its cost estimate for the
-first pass and by the work measured in it for the second; the next three
+first pass and by the evaluation time measured in it for the second; the next three
 passes each try exchanges of systems between XCDs that finished late and
 early, and the fastest assignment serves every pass after them (with
 --warmup 5, every timed pass).  --xcd-search -1 stops after the second

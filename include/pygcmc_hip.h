@@ -105,13 +105,14 @@ typedef struct gcmc_tuning {
                             before the first run and by the work measured in the last run after
                             it.  Changes which workgroups serve a system, never results */
     int xcd_search;      /* batched form, balanced: when runs repeat one step range (the
-                            snapshot restored between them), up to this many of them each try
-                            exchanges of systems between XCDs of the fastest assignment so far
-                            (the i-th XCD to finish last with the i-th to finish first, one
-                            exchange of two systems per pair, chosen from the systems' measured
-                            evaluation times); the fastest assignment is kept.  0 (default) = 3
-                            tries, -1 = none.  Changes which workgroups serve a system, never
-                            results */
+                            snapshot restored between them), the second is balanced by the
+                            evaluation time each system took in the first, and up to this many
+                            of the next each try exchanges of systems between XCDs of the
+                            fastest assignment so far (the i-th XCD to finish last with the
+                            i-th to finish first, one exchange of two systems per pair, chosen
+                            from the systems' measured evaluation times); the fastest
+                            assignment is kept.  0 (default) = 3 tries, -1 = none.  Changes
+                            which workgroups serve a system, never results */
 } gcmc_tuning;
 
 #define GCMC_MODE_PLAIN 0   /* one move per launch pair, systems one after another */

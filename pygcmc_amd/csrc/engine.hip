@@ -1265,13 +1265,14 @@ int gcmc_engine_run(gcmc_engine *E, int nsteps, int mode)
             X.tried.clear();
         }
         const int tries = E->tune.xcd_search == 0 ? 3 : std::max(E->tune.xcd_search, 0);
-        std::vector<double> w(R);
-        for (int r = 0; r < R; ++r) w[r] = E->reps[r]->last_work;
         std::vector<std::pair<int, int>> pr;
         if (X.runs == 1) {
             // first run of these steps (on the estimate's or an earlier range's
-            // assignment): next, the balance by this run's measured work
-            X.next = lpt_classes(E, w);
+            // assignment): next, the balance by this run's measured evaluation
+            // times (a replica's evaluation time depends on the replicas it
+            // shares an XCD with, so the balance is a candidate like the
+            // exchanges: the fastest assignment is kept)
+            X.next = lpt_classes(E, evt);
         } else if (X.runs - 1 <= tries && propose_exchange(E, X.next, pr)) {
             X.tried.insert(X.tried.end(), pr.begin(), pr.end());
         } else {
