@@ -429,29 +429,38 @@ __device__ inline int pair_pass(CRep &C, const Lds &L, int n, int nf, float &acc
             cmp += err;
         }
     };
-    // two atoms per thread per round, both loads issued before either is used
-    for (int t0 = threadIdx.x; t0 < A; t0 += 2 * NT) {
-        const int t1 = t0 + NT;
-        const bool two = t1 < A;
-        int g0, g1;
-        if (mapped) {
-            g0 = L.amap[t0];
-            g1 = two ? L.amap[t1] : g0;
-        } else {
-            bsearch_le2(L.incn, n, t0, two ? t1 : t0, g0, g1);
+    // U atoms per thread per round, all loads issued before any is used: a
+    // thread takes atoms t, t + NT, t + 2 NT, ... in that order (the order of
+    // its sums, whatever U is)
+    constexpr int U = 3;
+    for (int t0 = threadIdx.x; t0 < A; t0 += U * NT) {
+        int a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int tu = t0 + u * NT;
+            const int tq = tu < A ? tu : t0;
+            int g;
+            if (mapped) {
+                g = L.amap[tq];
+            } else {
+                int gx;
+                bsearch_le2(L.incn, n, tq, tq, g, gx);
+            }
+            a[u] = L.incs[g] + (tq - L.incn[g]);
         }
-        const int a0 = L.incs[g0] + (t0 - L.incn[g0]);
-        const int a1 = two ? L.incs[g1] + (t1 - L.incn[g1]) : a0;
-        const float4 at0 = atom[a0];
-        const int ty0 = atype[a0];
-        const float4 at1 = atom[a1];
-        const int ty1 = atype[a1];
-        if (c64) {
-            atom_terms64(at0, ty0);
-            if (two) atom_terms64(at1, ty1);
-        } else {
-            atom_terms(at0, ty0);
-            if (two) atom_terms(at1, ty1);
+        float4 at[U];
+        int ty[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            at[u] = atom[a[u]];
+            ty[u] = atype[a[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t0 + u * NT < A) {
+                if (c64) atom_terms64(at[u], ty[u]);
+                else atom_terms(at[u], ty[u]);
+            }
         }
     }
     __syncthreads();
