@@ -334,6 +334,15 @@ int gcmc_copy_probe(long long bytes, int copies);
 int gcmc_host_within(int n, const float *a, const float *b, const float *cryst, float cutoff,
                      unsigned char *out);
 
+/* Host check of the XCD assignment search (gcmc_tuning.xcd_search); no
+ * counterpart in the reference.  From the assignment best[n] (class of each
+ * system, nclass classes), the classes' finish times fin[nclass] and the
+ * systems' evaluation times evt[n]: next[n] = the assignment one try of the
+ * search runs (the i-th latest class paired with the i-th earliest, one
+ * exchange of two systems per pair), *nexchange = exchanges made. */
+int gcmc_host_xcd_exchange(int nclass, int n, const int *best, const double *fin, const double *evt, int *next,
+                           int *nexchange);
+
 /* Host check of the cell-list superset property for n (centre, group) pairs:
  * out[i] = 1 when the group's cell is one of the stencil cells the energy
  * kernel scans for the centre: a stencil cell around the centre's cell (box

@@ -839,32 +839,33 @@ std::vector<int> lpt_classes(const gcmc_engine *E, const std::vector<double> &w)
 int assign_classes(gcmc_engine *E, const std::vector<double> &w) { return upload_classes(E, lpt_classes(E, w)); }
 
 // One exchange between class S (finished later) and class F under the
-// fastest assignment so far, applied to next: a replica a of S for a replica
-// b of F, the pair whose evaluation times (each class's finish time taken as
-// proportional to its summed evaluation time) leave the later of the two
-// classes' predicted finishes earliest; none when no untried pair would end
-// S sooner.
-bool propose_one(const gcmc_engine *E, int S, int F, std::vector<int> &next, std::pair<int, int> &pair)
+// assignment best (class of each replica), applied to next: a replica a of S
+// for a replica b of F, the pair whose evaluation times evt (each class's
+// finish time fin taken as proportional to its summed evaluation time) leave
+// the later of the two classes' predicted finishes earliest; none when no
+// pair outside tried would end S sooner.
+bool xcd_exchange_one(const std::vector<int> &best, const std::vector<double> &fin, const std::vector<double> &evt,
+                      const std::vector<std::pair<int, int>> &tried, int S, int F, std::vector<int> &next,
+                      std::pair<int, int> &pair)
 {
-    const gcmc_engine::XcdSearch &X = E->xs;
-    const int R = (int)X.best.size();
+    const int R = (int)best.size();
     double eS = 0.0, eF = 0.0;
     for (int r = 0; r < R; ++r) {
-        if (X.best[r] == S) eS += X.evt[r];
-        if (X.best[r] == F) eF += X.evt[r];
+        if (best[r] == S) eS += evt[r];
+        if (best[r] == F) eF += evt[r];
     }
     if (eS <= 0.0 || eF <= 0.0) return false;
-    const double kS = X.fin[S] / eS, kF = X.fin[F] / eF;
-    double top = X.fin[S];
+    const double kS = fin[S] / eS, kF = fin[F] / eF;
+    double top = fin[S];
     int ba = -1, bb = -1;
     for (int a = 0; a < R; ++a) {
-        if (X.best[a] != S) continue;
+        if (best[a] != S) continue;
         for (int b = 0; b < R; ++b) {
-            if (X.best[b] != F) continue;
-            const double d = X.evt[a] - X.evt[b];
+            if (best[b] != F) continue;
+            const double d = evt[a] - evt[b];
             if (d <= 0.0) continue;
-            if (std::find(X.tried.begin(), X.tried.end(), std::make_pair(a, b)) != X.tried.end()) continue;
-            const double pred = std::max(X.fin[S] - kS * d, X.fin[F] + kF * d);
+            if (std::find(tried.begin(), tried.end(), std::make_pair(a, b)) != tried.end()) continue;
+            const double pred = std::max(fin[S] - kS * d, fin[F] + kF * d);
             if (pred < top) {
                 top = pred;
                 ba = a;
@@ -879,26 +880,32 @@ bool propose_one(const gcmc_engine *E, int S, int F, std::vector<int> &next, std
     return true;
 }
 
-// The exchanges of one try from the fastest assignment so far: the classes
-// ordered by finish time, the i-th latest paired with the i-th earliest, one
-// exchange per pair (propose_one).  A try changes several classes at once;
-// the run measures it as a whole and the faster assignment is kept.
-bool propose_exchange(const gcmc_engine *E, std::vector<int> &next, std::vector<std::pair<int, int>> &pairs)
+// The exchanges of one try from the assignment best: the classes ordered by
+// finish time, the i-th latest paired with the i-th earliest, one exchange
+// per pair (xcd_exchange_one).  A try changes several classes at once; the
+// run measures it as a whole and the faster assignment is kept.
+bool xcd_exchanges(int nclass, const std::vector<int> &best, const std::vector<double> &fin,
+                   const std::vector<double> &evt, const std::vector<std::pair<int, int>> &tried,
+                   std::vector<int> &next, std::vector<std::pair<int, int>> &pairs)
 {
-    const gcmc_engine::XcdSearch &X = E->xs;
-    const int nclass = E->xm.nclass;
     std::vector<int> order(nclass);
     for (int c = 0; c < nclass; ++c) order[c] = c;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return X.fin[a] > X.fin[b]; });
-    next = X.best;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return fin[a] > fin[b]; });
+    next = best;
     pairs.clear();
     for (int i = 0; i < nclass / 2; ++i) {
         const int S = order[i], F = order[nclass - 1 - i];
-        if (!(X.fin[S] > X.fin[F])) break;
+        if (!(fin[S] > fin[F])) break;
         std::pair<int, int> pr;
-        if (propose_one(E, S, F, next, pr)) pairs.push_back(pr);
+        if (xcd_exchange_one(best, fin, evt, tried, S, F, next, pr)) pairs.push_back(pr);
     }
     return !pairs.empty();
+}
+
+bool propose_exchange(const gcmc_engine *E, std::vector<int> &next, std::vector<std::pair<int, int>> &pairs)
+{
+    const gcmc_engine::XcdSearch &X = E->xs;
+    return xcd_exchanges(E->xm.nclass, X.best, X.fin, X.evt, X.tried, next, pairs);
 }
 
 // Which XCC ids the device's workgroups run on, and the assignment of
@@ -1791,6 +1798,22 @@ int gcmc_adjoint_backward_noalloc(int npose, int per_pose, int ncol, const float
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(GCMC_E_HIP, std::string("adjoint backward: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int gcmc_host_xcd_exchange(int nclass, int n, const int *best, const double *fin, const double *evt, int *next,
+                           int *nexchange)
+{
+    if (nclass < 1 || n < 0 || (n > 0 && (!best || !evt || !next)) || !fin || !nexchange)
+        return fail(GCMC_E_ARG, "bad arguments");
+    for (int r = 0; r < n; ++r)
+        if (best[r] < 0 || best[r] >= nclass) return fail(GCMC_E_ARG, "class outside [0, nclass)");
+    std::vector<int> b(best, best + n), nx;
+    std::vector<double> f(fin, fin + nclass), e(evt, evt + n);
+    std::vector<std::pair<int, int>> pairs;
+    xcd_exchanges(nclass, b, f, e, {}, nx, pairs);
+    std::copy(nx.begin(), nx.end(), next);
+    *nexchange = (int)pairs.size();
     return 0;
 }
 

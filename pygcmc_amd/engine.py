@@ -93,6 +93,7 @@ def lib():
         L.gcmc_engine_eval_poses.argtypes = [P, I, I, I, P, P, P, P, P, P]
         L.gcmc_host_within.argtypes = [I, P, P, P, ctypes.c_float, P]
         L.gcmc_host_stencil_covers.argtypes = [I, P, P, P, P, ctypes.c_float, ctypes.c_float, P]
+        L.gcmc_host_xcd_exchange.argtypes = [I, I, P, P, P, P, P]
         L.runGCMC_hip.argtypes = [P, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -135,6 +136,21 @@ def host_stencil_covers(centres, groups, cryst, startxyz, cutoff, cell_size=0.0)
     _check(lib().gcmc_host_stencil_covers(len(a), _p(a), _p(b), _p(c), _p(st), float(cutoff),
                                           float(cell_size), _p(out)))
     return out.astype(bool)
+
+
+def host_xcd_exchange(best, fin, evt):
+    """One try of the XCD assignment search (host check): from the class of
+    each system, the classes' finish times and the systems' evaluation times,
+    the assignment the try runs and the number of exchanges in it."""
+    b = np.ascontiguousarray(best, np.int32)
+    f = np.ascontiguousarray(fin, np.float64)
+    e = np.ascontiguousarray(evt, np.float64)
+    if len(e) != len(b):
+        raise ValueError('best and evt must have one entry per system')
+    nxt = np.zeros(len(b), np.int32)
+    nx = np.zeros(1, np.int32)
+    _check(lib().gcmc_host_xcd_exchange(len(f), len(b), _p(b), _p(f), _p(e), _p(nxt), _p(nx)))
+    return nxt, int(nx[0])
 
 
 class Engine:

@@ -158,3 +158,64 @@ def test_pruned_stencil_covers_cutoff_pairs_at_cell_faces(cell):
     cov = engine.host_stencil_covers(a, b, L, start, rc, cell)
     assert inc.sum() > 1000
     assert cov[inc].all()
+
+
+def _lpt(w, ncls):
+    order = sorted(range(len(w)), key=lambda r: -w[r])
+    load, cls = [0.0] * ncls, [0] * len(w)
+    for r in order:
+        c = min(range(ncls), key=lambda q: (load[q], q))
+        load[c] += w[r]
+        cls[r] = c
+    return np.array(cls, np.int32)
+
+
+@pytest.mark.parametrize('seed', [1, 2, 3])
+def test_xcd_exchange_try(seed):
+    """One try of the XCD assignment search (host code of the engine): the
+    i-th latest class trades one system with the i-th earliest, class sizes
+    stay, and each trade is the pair that, finishing times taken as
+    proportional to summed evaluation times, ends the later of the two
+    classes earliest -- and earlier than the slower class ended."""
+    rng = np.random.default_rng(seed)
+    ncls, n = 8, 64
+    evt = rng.uniform(5.0, 20.0, n)
+    best = _lpt(list(rng.uniform(0.5, 1.5, n) * evt), ncls)
+    # finish times: proportional to the class's evaluation time with a
+    # class-dependent rate (as measured: the rate varies between XCDs)
+    rate = rng.uniform(9.0, 11.0, ncls)
+    fin = np.array([rate[c] * evt[best == c].sum() for c in range(ncls)])
+    nxt, nx = engine.host_xcd_exchange(best, fin, evt)
+    assert np.array_equal(np.bincount(nxt, minlength=ncls), np.bincount(best, minlength=ncls))
+    order = sorted(range(ncls), key=lambda c: -fin[c])
+    partner = {}
+    for i in range(ncls // 2):
+        partner[order[i]] = order[ncls - 1 - i]
+        partner[order[ncls - 1 - i]] = order[i]
+    moved = np.nonzero(nxt != best)[0]
+    assert len(moved) == 2 * nx and nx >= 1
+    for r in moved:
+        assert nxt[r] == partner[best[r]]
+    for i in range(ncls // 2):
+        S, F = order[i], order[ncls - 1 - i]
+        kS, kF = fin[S] / evt[best == S].sum(), fin[F] / evt[best == F].sum()
+        cand = [(max(fin[S] - kS * (evt[a] - evt[b]), fin[F] + kF * (evt[a] - evt[b])), a, b)
+                for a in np.nonzero(best == S)[0] for b in np.nonzero(best == F)[0] if evt[a] > evt[b]]
+        good = [c for c in cand if c[0] < fin[S]]
+        a_moved = [r for r in moved if best[r] == S]
+        if not good:
+            assert not a_moved
+            continue
+        pred, a, b = min(good, key=lambda c: c[0])
+        assert a_moved == [a] and nxt[a] == F and nxt[b] == S
+
+
+def test_xcd_exchange_leaves_balanced_and_single_class_alone():
+    evt = np.linspace(1.0, 8.0, 16)
+    best = np.arange(16, dtype=np.int32) % 4
+    nxt, nx = engine.host_xcd_exchange(best, np.full(4, 100.0), evt)
+    assert nx == 0 and np.array_equal(nxt, best)
+    nxt, nx = engine.host_xcd_exchange(np.zeros(16, np.int32), np.array([50.0]), evt)
+    assert nx == 0
+    with pytest.raises(RuntimeError):
+        engine.host_xcd_exchange(np.full(16, 4, np.int32), np.full(4, 1.0), evt)
