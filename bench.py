@@ -380,7 +380,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3, help='timed passes of the whole workload')
-    ap.add_argument('--warmup', type=int, default=1, help='untimed passes before them')
+    ap.add_argument('--warmup', type=int, default=5,
+                    help='untimed passes before them (5: the XCD assignment search ends within them)')
     ap.add_argument('--replicas', type=int, default=64)
     ap.add_argument('--moves', type=int, default=10000, help='moves per system in one pass')
     ap.add_argument('--plain-moves', type=int, default=200)
