@@ -249,7 +249,8 @@ def measured_traffic(moves, replicas, tuning):
             int(rec.get('wg_size', -1)) == int(tuning.get('wg_size', 0) or 256) and
             int(rec.get('pair_lds', 1)) == int(tuning.get('pair_lds', 1)) and
             int(rec.get('fp32_coulomb', 0)) == int(tuning.get('fp32_coulomb', 0)) and
-            int(rec.get('xcd_round_robin', -1)) == int(tuning.get('xcd_round_robin', 0)))
+            int(rec.get('xcd_round_robin', -1)) == int(tuning.get('xcd_round_robin', 0)) and
+            int(rec.get('xcd_search', -2)) == int(tuning.get('xcd_search', 0)))
     if not same:
         return None, None
     return float(rec['traffic_bytes']), 'profiles/k_batched_64replicas.pmc.txt (rocprofv3 --pmc, same workload)'
