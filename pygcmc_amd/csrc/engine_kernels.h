@@ -1718,71 +1718,74 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             // may hit this CU's L1, which only a claiming workgroup invalidates,
             // so idle workgroups would keep seeing an old word.  Polls are
             // loads; an atomic is issued only to claim (read-modify-writes of
-            // one address serialize).
+            // one address serialize).  The scheduler's words (progress,
+            // finished, abort) are read only when nothing could be claimed:
+            // a workgroup that finds work goes from its last item to its next
+            // with two memory round trips (the claim words, the claim).
             const int lane = tid;
             int found = -1, item = 0;
-            int ctl[3] = {0, 0, 0};
-            if (lane == 0) {
-                ctl[0] = __hip_atomic_load(&sch->progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ctl[1] = __hip_atomic_load(&sch->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ctl[2] = __hip_atomic_load(&sch->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            const int prog = __shfl(ctl[0], 0, 64), fin = __shfl(ctl[1], 0, 64), ab = __shfl(ctl[2], 0, 64);
-            if (prog != seen) {
-                seen = prog;
-                idle = 0;
-            }
-            if (fin >= R || ab != 0) {
-                found = -2;
-            } else {
-                for (int b0 = 0; b0 < nc && found < 0; b0 += 64) {
-                    int r = -1;
-                    unsigned long long w = 0ull;
-                    if (b0 + lane < nc) {
-                        r = xm.cls_reps[c0 + (rot + b0 + lane) % nc];
-                        w = __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int b0 = 0; b0 < nc && found < 0; b0 += 64) {
+                int r = -1;
+                unsigned long long w = 0ull;
+                if (b0 + lane < nc) {
+                    r = xm.cls_reps[c0 + (rot + b0 + lane) % nc];
+                    w = __hip_atomic_load(&st[r].claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                bool avail = r >= 0 && (w & 0xFFFFFull) < ((w >> 20) & 0xFFFFFull);
+                // Moves with few unclaimed items first (up to 4, then up to
+                // 16): a replica waiting on one or a few evaluations is on its
+                // own critical chain.  Within those classes and among the
+                // larger moves, the replica with the most estimated work left
+                // first (claim_prio): the run ends with its last replica, and
+                // a large move occupies the XCD's workgroups for several
+                // rounds whichever order they come in.  (Fewest unclaimed
+                // items first among the large moves too: 5 % slower.)
+                const unsigned long long unc = ((w >> 20) & 0xFFFFFull) - (w & 0xFFFFFull);
+                unsigned size = unc <= 4ull ? 0u : (unc <= 16ull ? 1u : 2u);
+                // affinity (aff > 0): a move of at least aff unclaimed items
+                // of the replica whose tables this workgroup holds in LDS
+                // comes before the other such moves (its species table and
+                // staged pair table are reused)
+                if (aff > 0 && unc >= (unsigned long long)aff) size = (r == cached_rep) ? 2u : 3u;
+                const unsigned prio = (unsigned)(w >> 40) >> 8;
+                while (__ballot(avail) != 0ull) {
+                    const unsigned key =
+                        wave_min(avail ? ((size << 22) | (prio << 6) | (unsigned)lane) : 0xFFFFFFFFu);
+                    const int l = (int)(key & 63u);
+                    const int rr = __shfl(r, l, 64);
+                    unsigned long long old = 0ull;
+                    if (lane == l)
+                        old = __hip_atomic_fetch_add(&st[rr].claim, 1ull, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                    old = __shfl(old, l, 64);
+                    const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
+                    if (c < n) {
+                        found = rr;
+                        item = c;
+                        break;
                     }
-                    bool avail = r >= 0 && (w & 0xFFFFFull) < ((w >> 20) & 0xFFFFFull);
-                    // Moves with few unclaimed items first (up to 4, then up to
-                    // 16): a replica waiting on one or a few evaluations is on its
-                    // own critical chain.  Within those classes and among the
-                    // larger moves, the replica with the most estimated work left
-                    // first (claim_prio): the run ends with its last replica, and
-                    // a large move occupies the XCD's workgroups for several
-                    // rounds whichever order they come in.  (Fewest unclaimed
-                    // items first among the large moves too: 5 % slower.)
-                    const unsigned long long unc = ((w >> 20) & 0xFFFFFull) - (w & 0xFFFFFull);
-                    unsigned size = unc <= 4ull ? 0u : (unc <= 16ull ? 1u : 2u);
-                    // affinity (aff > 0): a move of at least aff unclaimed items
-                    // of the replica whose tables this workgroup holds in LDS
-                    // comes before the other such moves (its species table and
-                    // staged pair table are reused)
-                    if (aff > 0 && unc >= (unsigned long long)aff) size = (r == cached_rep) ? 2u : 3u;
-                    const unsigned prio = (unsigned)(w >> 40) >> 8;
-                    while (__ballot(avail) != 0ull) {
-                        const unsigned key =
-                            wave_min(avail ? ((size << 22) | (prio << 6) | (unsigned)lane) : 0xFFFFFFFFu);
-                        const int l = (int)(key & 63u);
-                        const int rr = __shfl(r, l, 64);
-                        unsigned long long old = 0ull;
-                        if (lane == l)
-                            old = __hip_atomic_fetch_add(&st[rr].claim, 1ull, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-                        old = __shfl(old, l, 64);
-                        const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
-                        if (c < n) {
-                            found = rr;
-                            item = c;
-                            break;
-                        }
-                        if (lane == l) avail = false;
-                    }
+                    if (lane == l) avail = false;
                 }
             }
-            if (found == -1 && ++idle > spin_limit) {
-                // no replica advanced during spin_limit polls: stop instead of hanging
-                if (lane == 0) __hip_atomic_store(&sch->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                found = -2;
+            if (found == -1) {
+                int ctl[3] = {0, 0, 0};
+                if (lane == 0) {
+                    ctl[0] = __hip_atomic_load(&sch->progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ctl[1] = __hip_atomic_load(&sch->finished, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ctl[2] = __hip_atomic_load(&sch->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                const int prog = __shfl(ctl[0], 0, 64), fin = __shfl(ctl[1], 0, 64), ab = __shfl(ctl[2], 0, 64);
+                if (prog != seen) {
+                    seen = prog;
+                    idle = 0;
+                }
+                if (fin >= R || ab != 0) {
+                    found = -2;
+                } else if (++idle > spin_limit) {
+                    // no replica advanced during spin_limit polls: stop instead of hanging
+                    if (lane == 0) __hip_atomic_store(&sch->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    found = -2;
+                }
             }
             if (lane == 0) {
                 L.misc[M_FOUND] = found;
