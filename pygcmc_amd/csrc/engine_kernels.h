@@ -62,6 +62,7 @@ enum Misc : int {
     M_ACC = 80,      // accept: decision / count of accepted
     M_NG = 81,       //         groups
     M_N = 82,        //         species count after the move
+    M_CNT = 83,      // batched: items claimed (consecutive, from M_ITEM)
     M_FOUND = 88,    // batched: replica claimed (-1 none, -2 exit)
     M_ITEM = 89,     //          item claimed
     M_LAST = 90,     //          this workgroup finished the move's last item
@@ -1561,6 +1562,24 @@ __device__ inline int prep_step(CRep &C, State &S, const Lds &L)
     return prep_step<NT>(C, S, L, false, 0, 0);
 }
 
+// Items a workgroup takes with one claim of a move of `unc` unclaimed items
+// in size class sz (0: at most 4, 1: 5-16, 2: more).  Consecutive items of one
+// move share the claim's round trips and acquire (an L1 invalidate), and the
+// workgroup's second and later evaluations find the move's data in L1: with
+// eight systems per XCD, a pass of the 64-system workload takes 700 instead of
+// 1150 ms with 4 items per claim of the largest moves and 2 of the middle
+// ones.  With few systems per XCD (nc < 4) the workgroups would idle while a
+// few of them work through a move's items one after another, so a claim then
+// takes no more than a fair share, ceil(unc / workgroups of the XCD).  Small
+// moves, a replica's critical path, take one item per claim.
+__device__ inline int claim_take(int sz, int unc, int nc, int nwx)
+{
+    const int want = sz == 2 ? 4 : (sz == 1 ? 2 : 1);
+    if (nc >= 4) return want;
+    const int share = (unc + nwx - 1) / nwx;
+    return min(want, max(share, 1));
+}
+
 // XCD-local ordering.  Every workgroup touching a replica in the batched form
 // runs on the same XCD, whose L2 is shared and whose L1s are write-through: a
 // release waits until this thread's memory operations reached L2; an acquire
@@ -1708,6 +1727,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
     const int c0 = xm.cls_start[cls], nc = xm.cls_start[cls + 1] - c0;
     if (nc <= 0) return;
     const int rot = (int)(blockIdx.x / (unsigned)xm.nclass) % nc;
+    const int nwx = max((int)(gridDim.x / (unsigned)xm.nclass), 1);   // workgroups per XCD
     int idle = 0, seen = -1;
     // replica whose species table, and pair table (Rep::tab) that are in LDS
     int cached_rep = -1, cached_tab = -1;
@@ -1723,7 +1743,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             // a workgroup that finds work goes from its last item to its next
             // with two memory round trips (the claim words, the claim).
             const int lane = tid;
-            int found = -1, item = 0;
+            int found = -1, item = 0, cnt = 1;
             for (int b0 = 0; b0 < nc && found < 0; b0 += 64) {
                 int r = -1;
                 unsigned long long w = 0ull;
@@ -1748,20 +1768,23 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
                 // staged pair table are reused)
                 if (aff > 0 && unc >= (unsigned long long)aff) size = (r == cached_rep) ? 2u : 3u;
                 const unsigned prio = (unsigned)(w >> 40) >> 8;
+                const int take_lane = claim_take((int)size, (int)unc, nc, nwx);
                 while (__ballot(avail) != 0ull) {
                     const unsigned key =
                         wave_min(avail ? ((size << 22) | (prio << 6) | (unsigned)lane) : 0xFFFFFFFFu);
                     const int l = (int)(key & 63u);
                     const int rr = __shfl(r, l, 64);
+                    const int take = __shfl(take_lane, l, 64);
                     unsigned long long old = 0ull;
                     if (lane == l)
-                        old = __hip_atomic_fetch_add(&st[rr].claim, 1ull, __ATOMIC_RELAXED,
+                        old = __hip_atomic_fetch_add(&st[rr].claim, (unsigned long long)take, __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT);
                     old = __shfl(old, l, 64);
                     const int c = (int)(old & 0xFFFFFull), n = (int)((old >> 20) & 0xFFFFFull);
                     if (c < n) {
                         found = rr;
                         item = c;
+                        cnt = min(take, n - c);
                         break;
                     }
                     if (lane == l) avail = false;
@@ -1790,12 +1813,14 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             if (lane == 0) {
                 L.misc[M_FOUND] = found;
                 L.misc[M_ITEM] = item;
+                L.misc[M_CNT] = cnt;
             }
         }
         __syncthreads();
         // uniform across the workgroup: the replica's fields become scalar loads
         const int found = __builtin_amdgcn_readfirstlane(L.misc[M_FOUND]);
         const int item = __builtin_amdgcn_readfirstlane(L.misc[M_ITEM]);
+        const int cnt = __builtin_amdgcn_readfirstlane(L.misc[M_CNT]);
         __syncthreads();
         if (found == -2) break;
         if (found == -1) {
@@ -1811,15 +1836,21 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
         const unsigned long long te0 = wall_clock64();
         const int tab = C.tab;
         ItemStats ist;
-        eval_item<NT, STAGED, true, false>(C, S, item, L, found != cached_rep, tab != cached_tab, &ist);
+        // the claimed items one after another; the last one's statistics are
+        // added after the completion count (the others' before it)
+        for (int j = 0; j < cnt; ++j) {
+            if (j > 0) __syncthreads();
+            eval_item<NT, STAGED, true, false>(C, S, item + j, L, j == 0 && found != cached_rep,
+                                               j == 0 && tab != cached_tab, j + 1 < cnt ? nullptr : &ist);
+        }
         cached_rep = found;
         cached_tab = tab;
         const unsigned long long te1 = wall_clock64();
         xcd_release();
         __syncthreads();
         if (tid == 0) {
-            const int old = __hip_atomic_fetch_add(&S.done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            L.misc[M_LAST] = (old == S.nitems - 1) ? 1 : 0;
+            const int old = __hip_atomic_fetch_add(&S.done, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            L.misc[M_LAST] = (old + cnt == S.nitems) ? 1 : 0;
             // statistics after the completion count: the release above does
             // not wait for them
             add_item_stats<true>(S, ist);
