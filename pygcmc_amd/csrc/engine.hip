@@ -170,6 +170,7 @@ struct gcmc_engine {
     int nt = 256;              // threads per workgroup
     XcdMap xm{};
     int *d_cls = nullptr;      // cls_start[nclass+1] then cls_reps[R]
+    unsigned long long *d_cu_inv = nullptr;  // XcdMap::cu_inv
     State *d_state_snap = nullptr;  // State of every replica at gcmc_engine_snapshot
     std::vector<int> cls_host;      // the replica -> XCD class map on the device (cls_start, cls_reps)
     std::vector<int> cls_of;        // the same, class of each replica
@@ -930,6 +931,9 @@ int build_xcd_map(gcmc_engine *E, int ncu)
     if (m.nclass == 0) return fail(GCMC_E_DEVICE, "XCC probe returned no ids");
     const int R = (int)E->reps.size();
     HIPX(hipMalloc(&E->d_cls, (m.nclass + 1 + R) * sizeof(int)));
+    HIPX(hipMalloc(&E->d_cu_inv, CU_SLOTS * sizeof(unsigned long long)));
+    HIPX(hipMemset(E->d_cu_inv, 0, CU_SLOTS * sizeof(unsigned long long)));
+    m.cu_inv = E->d_cu_inv;
     m.cls_start = E->d_cls;
     m.cls_reps = E->d_cls + m.nclass + 1;
     std::vector<double> w(R);
@@ -1016,6 +1020,7 @@ int gcmc_engine_destroy(gcmc_engine *E)
     if (E->d_state) hipFree(E->d_state);
     if (E->d_sched) hipFree(E->d_sched);
     if (E->d_cls) hipFree(E->d_cls);
+    if (E->d_cu_inv) hipFree(E->d_cu_inv);
     if (E->d_state_snap) hipFree(E->d_state_snap);
     if (E->d_adj_work) hipFree(E->d_adj_work);
     if (E->ev0) hipEventDestroy(E->ev0);

@@ -63,6 +63,7 @@ enum Misc : int {
     M_NG = 81,       //         groups
     M_N = 82,        //         species count after the move
     M_CNT = 83,      // batched: items claimed (consecutive, from M_ITEM)
+    M_INV = 84,      //          the claim needs an L1 invalidate (cu_acquire_needed)
     M_FOUND = 88,    // batched: replica claimed (-1 none, -2 exit)
     M_ITEM = 89,     //          item claimed
     M_LAST = 90,     //          this workgroup finished the move's last item
@@ -1632,8 +1633,13 @@ __device__ inline void advance_and_publish(CRep &C, State &S, Sched *sch, const 
             }
             xcd_release();
             __syncthreads();
-            if (threadIdx.x == 0)
+            if (threadIdx.x == 0) {
+                // every thread's writes have completed: a CU whose L1 was
+                // invalidated after this time holds no older copy of them
+                S.t_data = wall_clock64();
+                xcd_release();
                 __hip_atomic_store(&S.claim, claim_word(prio, nitems, 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             break;
         }
         accept_step<NT, false>(C, S, L);
@@ -1706,9 +1712,32 @@ struct XcdMap {
     int xcc_class[16];
     const int *cls_start;
     const int *cls_reps;
+    unsigned long long *cu_inv;   // [CU_SLOTS] time of each CU's last L1 invalidate (100 MHz ticks)
 };
 
 __device__ inline int my_xcc() { return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (15 << 11)) & 0xf); }
+
+// This CU's slot in XcdMap::cu_inv: XCC id, shader engine, shader array and
+// CU id from the XCC_ID and HW_ID registers (distinct for distinct CUs).
+constexpr int CU_SLOTS = 4096;
+__device__ inline int my_cu_slot()
+{
+    const unsigned hw = __builtin_amdgcn_s_getreg((4) | (0 << 6) | (31 << 11));
+    const unsigned cu = (hw >> 8) & 15u, sh = (hw >> 12) & 1u, se = (hw >> 13) & 7u;
+    return (int)(((unsigned)my_xcc() << 8) | (se << 5) | (sh << 4) | cu);
+}
+
+// The acquire before evaluating a claimed move's items is a device-scope L1
+// invalidate, which also empties the L1 of the CU's other workgroups.  It is
+// needed only when the CU's L1 may hold lines older than the move's data:
+// not when this CU's last invalidate came after the move's data was complete
+// (State::t_data).  One thread asks; both times are read at device scope.
+__device__ inline int cu_acquire_needed(const State &S, const unsigned long long *cu_inv, int slot)
+{
+    const unsigned long long td = __hip_atomic_load(&S.t_data, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long last = __hip_atomic_load(&cu_inv[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return last >= td ? 0 : 1;
+}
 
 // persistent form: workgroups of an XCD claim evaluations from that XCD's
 // replicas; the workgroup completing a replica's last evaluation runs its
@@ -1728,6 +1757,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
     if (nc <= 0) return;
     const int rot = (int)(blockIdx.x / (unsigned)xm.nclass) % nc;
     const int nwx = max((int)(gridDim.x / (unsigned)xm.nclass), 1);   // workgroups per XCD
+    const int slot = my_cu_slot();
     int idle = 0, seen = -1;
     // replica whose species table, and pair table (Rep::tab) that are in LDS
     int cached_rep = -1, cached_tab = -1;
@@ -1814,6 +1844,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
                 L.misc[M_FOUND] = found;
                 L.misc[M_ITEM] = item;
                 L.misc[M_CNT] = cnt;
+                L.misc[M_INV] = found >= 0 ? cu_acquire_needed(st[found], xm.cu_inv, slot) : 1;
             }
         }
         __syncthreads();
@@ -1821,6 +1852,7 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
         const int found = __builtin_amdgcn_readfirstlane(L.misc[M_FOUND]);
         const int item = __builtin_amdgcn_readfirstlane(L.misc[M_ITEM]);
         const int cnt = __builtin_amdgcn_readfirstlane(L.misc[M_CNT]);
+        const int need_inv = __builtin_amdgcn_readfirstlane(L.misc[M_INV]);
         __syncthreads();
         if (found == -2) break;
         if (found == -1) {
@@ -1830,7 +1862,15 @@ __global__ __launch_bounds__(NT, 768 / NT) void k_batched(const Rep *reps, State
             __builtin_amdgcn_s_sleep(32);
             continue;
         }
-        xcd_acquire();
+        if (need_inv) {
+            const unsigned long long tinv = wall_clock64();
+            xcd_acquire();
+            if (tid == 0) {
+                // recorded once this wave's invalidate has executed
+                xcd_release();
+                __hip_atomic_store(&xm.cu_inv[slot], tinv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
         CRep &C = rep_of(reps, found);
         State &S = st[found];
         const unsigned long long te0 = wall_clock64();

@@ -138,6 +138,9 @@ struct State {
     // and between a move's publication and its last evaluation finishing
     unsigned long long t_eval, t_accept, t_prep, t_step;
     unsigned long long t_pub;
+    // 100 MHz time by which every write the current move's evaluations read
+    // (prep, the previous move's commit) had reached L2; set when it is published
+    unsigned long long t_data;
     // sub-phases of evaluations (ticks, summed): pose+adjacency, stencil
     // count+scan, candidate tests, pair passes
     unsigned long long t_sub[4];
