@@ -805,10 +805,13 @@ __device__ inline void eval_item(CRep &C, const State &S, int item, const Lds &L
     const int nf = F.nf;
     const auto tc = gbl(C.tc);
     const auto atom = gbl(C.atom);
+    // translate / rotate: item 2t is trial t's old pose, 2t + 1 its new pose;
+    // a claim of consecutive items evaluates both, the second reading the
+    // neighbourhood the first left in L1
     int t = item, phase = 0;
-    if (kind >= K_TRN && item >= k) {
-        t = item - k;
-        phase = 1;
+    if (kind >= K_TRN) {
+        t = item >> 1;
+        phase = item & 1;
     }
     // the species table stays in LDS while a workgroup keeps evaluating
     // items of one replica (stage_rep = false), the pair table while it
