@@ -86,11 +86,11 @@ typedef struct gcmc_tuning {
     float cell_size;     /* target edge (A) of the group-centre cell list; 0 = 5 A */
     int persistent_wgs;  /* workgroups of the batched kernel; 0 = per CU by wg_size */
     int bucket_cap;      /* mobile groups per cell before overflow; 0 = auto */
-    int pair_lds;        /* 1 = hold the system's LJ pair table (X x Y entries, up to 16 KiB)
-                            in each workgroup's LDS, copied with global_load_lds when a
-                            workgroup turns to a system with another table; 0 (default) =
-                            read it from global memory, faster on the 64-replica
-                            benchmark; both give identical results */
+    int pair_lds;        /* 0 (default) or 1 = hold the system's LJ pair table (X x Y
+                            entries) in each workgroup's LDS when it fits in 16 KiB, copied
+                            with global_load_lds when a workgroup turns to a system with
+                            another table, faster on the 64-replica benchmark; -1 = read
+                            it from global memory; both give identical results */
     int wg_size;         /* threads per workgroup: 256, 512 or 1024; 0 = 256 */
     int fp32_coulomb;    /* 1 = Coulomb pair terms in FP32 like the LJ terms; 0 (default) =
                             Coulomb in FP64 from the exactly formed FP32 displacement */

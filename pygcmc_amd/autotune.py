@@ -2,7 +2,7 @@
 
 The engine exposes tuning knobs (``gcmc_tuning`` in ``include/pygcmc_hip.h``):
 cell-list edge, threads per workgroup, number of persistent workgroups,
-mobile-bucket capacity and whether the pair table is staged in LDS.  ``autotune`` times a
+mobile-bucket capacity and whether the pair table is staged in LDS (-1: no).  ``autotune`` times a
 short batched run of the caller's systems for each candidate setting -- on
 copies of the arrays, so the caller's state is untouched -- and returns the
 fastest setting; ``Engine(**best)`` then runs the real workload with it.
@@ -29,7 +29,7 @@ DEFAULT_SPACE = {
     'cell_size': (5.0, 6.0, 7.5, 9.0),
     'wg_size': (256, 512),
     'persistent_wgs': (0,),
-    'pair_lds': (0, 1),
+    'pair_lds': (-1, 0),
 }
 
 

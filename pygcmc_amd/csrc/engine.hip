@@ -609,7 +609,7 @@ int compute_layout(gcmc_engine *E)
         xy = std::max(xy, H->rep.X * H->rep.Y);
     }
     const int pair_bytes = 16 * xy;
-    E->staged = E->tune.pair_lds != 0 && pair_bytes <= 16384;
+    E->staged = E->tune.pair_lds >= 0 && pair_bytes <= 16384;
     auto up = [](int x) { return (x + 15) & ~15; };
     LdsLayout &l = E->lay;
     l.inc_cap = INC_CAP;

@@ -163,7 +163,7 @@ def test_pair_rows_in_lds_identical():
     chains, in the batched and plain forms."""
     s = systems.multi(nsteps=300, seed=13)
     outs = []
-    for pair_lds in (0, 1):
+    for pair_lds in (-1, 0):
         for mode in ('batched', 'plain'):
             g = systems.copy(s)
             eng = Engine(pair_lds=pair_lds)
